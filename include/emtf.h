@@ -1,0 +1,105 @@
+/*
+ * emtf.h -- C ABI of the MI355X island-model evolutionary-multitasking engine.
+ *
+ * The reference (MZT-DW/SG-EMTF, proposed model/) has no library interface:
+ * its "API" is a set of compile-time knobs and kernel launches made from one
+ * host thread per task.  Each entry point below names the reference code it
+ * replaces.  All functions return 0 (EMTF_OK) or a negative EMTF_E_* code;
+ * emtf_last_error() gives a message for the calling thread.  Host pointers are
+ * copied; device buffers are owned by the handle.  A handle is used by one host
+ * thread at a time; independent handles may run concurrently on different
+ * streams.  Plain C types only.
+ */
+#ifndef EMTF_H
+#define EMTF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMTF_OK 0
+#define EMTF_E_INVALID (-1)     /* a descriptor or argument is out of range */
+#define EMTF_E_DEVICE (-2)      /* the HIP runtime reported an error */
+#define EMTF_E_NOMEM (-3)       /* device allocation failed */
+#define EMTF_E_UNSUPPORTED (-4) /* shape outside what the built kernels cover */
+
+/* Objectives: ids 0..6 are the reference's evaluation() cases
+ * (proposed model/SG-EMTF_DE.cu:857-1061); 7 is the CEC Elliptic. */
+enum {
+    EMTF_SPHERE = 0, EMTF_ROSENBROCK = 1, EMTF_ACKLEY = 2, EMTF_RASTRIGIN = 3,
+    EMTF_GRIEWANK = 4, EMTF_WEIERSTRASS = 5, EMTF_SCHWEFEL = 6, EMTF_ELLIPTIC = 7
+};
+/* z = x - o | z = Q (x - o) | z = M (x - o)  (reference: z = M (x + b), SG-EMTF_DE.cu:859) */
+enum { EMTF_SHIFT = 0, EMTF_ROTATE = 1, EMTF_DIAG = 2 };
+/* DE/rand/1/bin | DE/current/1/bin (reference CrossOver+Mutation, SG-EMTF_DE.cu:474-677)
+ * | SBX + polynomial mutation (reference GA, SG-EMTF.cu:548-831) */
+enum { EMTF_DE_RAND_1_BIN = 0, EMTF_DE_CURRENT_1_BIN = 1, EMTF_GA_SBX_PM = 2 };
+
+/* One continuous task.  Replaces the per-task globals task_type / M / b /
+ * Range (SG-EMTF.cu:92-124, rangeInit :197-214, para_init :181-195). */
+typedef struct emtf_task_desc {
+    int32_t func;        /* EMTF_SPHERE .. EMTF_ELLIPTIC */
+    int32_t dim;         /* 1 .. 128 */
+    int32_t pop;         /* individuals, a multiple of n_islands */
+    int32_t xform;       /* EMTF_SHIFT / EMTF_ROTATE / EMTF_DIAG */
+    float lo, hi;        /* search box, every dimension */
+    const float* rot;    /* dim*dim row-major Q (EMTF_ROTATE), else NULL */
+    const float* shift;  /* dim, o (NULL = 0) */
+    const float* diag;   /* dim, M (EMTF_DIAG), else NULL */
+} emtf_task_desc;
+
+/* Engine knobs.  Replaces the #define block SG-EMTF.cu:28-68 / SG-EMTF_DE.cu:23-59:
+ * F, CROSSOVER_RATE, ETA, DM, ISLAND_NUM/INDIV_PERBLOCK, INTERVAL_MIGRA,
+ * MIGRA_NUM, INTERVAL_TRANSFER, TRANSFER_NUM. */
+typedef struct emtf_params {
+    int32_t algo;               /* EMTF_DE_RAND_1_BIN / EMTF_DE_CURRENT_1_BIN / EMTF_GA_SBX_PM */
+    float F, CR;                /* DE scale and binomial crossover rate */
+    float eta_c, eta_m, pm;     /* GA: SBX index, mutation index, per-gene mutation rate */
+    int32_t n_islands;          /* sub-populations per task */
+    int32_t exchange_interval;  /* cross-island resampling period in generations (0 = off) */
+    int32_t migrate_interval;   /* intra-task island migration period (0 = off) */
+    int32_t migrate_count;      /* individuals per island pair, <= island size / 2 */
+    int32_t transfer_interval;  /* inter-task elite transfer period (0 = off) */
+    int32_t transfer_count;     /* elites per transfer, <= pop / 2 */
+    float transfer_cr;          /* keep own gene when u <= transfer_cr; 0 takes every donor gene */
+    uint64_t seed;
+} emtf_params;
+
+typedef struct emtf_handle emtf_handle;
+
+/* Allocate device state, create generation 0 and evaluate it.
+ * Replaces initialization() (SG-EMTF.cu:293-380) + pop_init/evaluate (:146, :1428). */
+int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params* params, emtf_handle** out);
+
+/* Advance every task by `generations` (asynchronous on `stream`, a hipStream_t;
+ * NULL = default stream).  Replaces iter()'s loop of GeneticOpera launches and
+ * the knowledgeTransfer handshake (SG-EMTF.cu:1646-1745, kernel :1289). */
+int emtf_run(emtf_handle* h, int32_t generations, void* stream);
+
+/* Wait for the work queued by emtf_run. */
+int emtf_sync(emtf_handle* h);
+
+/* Generations completed so far. */
+int emtf_generation(const emtf_handle* h, int32_t* generation);
+
+/* Copy one task's population x[pop][dim] and fitness[pop] to host memory
+ * (synchronises).  Replaces popTransfer (SG-EMTF.cu:216-229). */
+int emtf_get_population(emtf_handle* h, int32_t task, float* x, float* fitness);
+
+/* Overwrite one task's population and fitness from host memory (synchronises). */
+int emtf_set_population(emtf_handle* h, int32_t task, const float* x, const float* fitness);
+
+/* Best fitness of every task and its genome (x_best rows of stride x_stride >= max dim).
+ * Replaces popSort + messageTransfer (SG-EMTF.cu:462-486, :1601-1615). */
+int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_stride);
+
+int emtf_destroy(emtf_handle* h);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* emtf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

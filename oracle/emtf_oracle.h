@@ -1,0 +1,118 @@
+/*
+ * emtf_oracle.h -- CPU oracle for the island-model evolutionary-multitasking
+ * engine.  TEST INFRASTRUCTURE ONLY: the product never includes, links or
+ * calls anything under oracle/.  Only tests/, the smoke check and the CPU
+ * baseline leg of the benchmark load liboracle.so, and only as the checker or
+ * as the timed CPU baseline.
+ *
+ * The oracle is a plain-C restatement of the reference operators
+ * (proposed model/SG-EMTF.cu, SG-EMTF_DE.cu), with the reference's cuRAND
+ * XORWOW streams replaced by counter-addressed Philox4x32-10 draws and its
+ * CUDA libm calls replaced by the FP32 function definitions below, so that a
+ * GPU implementation following the same arithmetic reproduces every bit.
+ *
+ * ---------------------------------------------------------------------------
+ * Arithmetic rules (shared contract with the device code)
+ *   - IEEE binary32, round-to-nearest-even, no contraction: every a*b+c is two
+ *     roundings unless written fmaf(a,b,c).  Build with -ffp-contract=off.
+ *   - Division and sqrtf are correctly rounded.
+ *   - The rotation z = Q w is the fmaf chain acc = fmaf(Q[j][d], w[d], acc)
+ *     for d = 0 .. Dp-1 (Dp = D rounded up to even, padding zeros), starting
+ *     from +0.  This is exactly what a sequence of 32x32x2 FP32 MFMA steps
+ *     computes on gfx950.
+ *   - Per-dimension terms t_j are summed in two accumulators: terms whose
+ *     block index j/4 is even go to A, odd to B, each in increasing j, both
+ *     starting at +0; the sum is A + B.
+ *
+ * Random draws: Philox4x32-10 (Salmon et al., SC'11) with
+ *   key     = { seed & 0xffffffff, seed >> 32 }
+ *   counter = { block, purpose<<24 | sub<<16 | index, generation, task }
+ * Every draw is addressed by (task, generation, purpose, index, sub, word),
+ * never by a thread id, so any thread mapping consumes identical numbers.
+ * ---------------------------------------------------------------------------
+ */
+#ifndef EMTF_ORACLE_H
+#define EMTF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OR_FUNC_SPHERE = 0, OR_FUNC_ROSENBROCK = 1, OR_FUNC_ACKLEY = 2, OR_FUNC_RASTRIGIN = 3,
+    OR_FUNC_GRIEWANK = 4, OR_FUNC_WEIERSTRASS = 5, OR_FUNC_SCHWEFEL = 6, OR_FUNC_ELLIPTIC = 7,
+    OR_NUM_FUNCS = 8
+};
+enum { OR_XFORM_SHIFT = 0, OR_XFORM_ROTATE = 1, OR_XFORM_DIAG = 2 };
+enum { OR_ALGO_DE_RAND_1_BIN = 0, OR_ALGO_DE_CURRENT_1_BIN = 1, OR_ALGO_GA_SBX_PM = 2 };
+enum {
+    OR_P_INIT = 1, OR_P_DE_IDX = 2, OR_P_DE_CR = 3, OR_P_GA_MATE = 4, OR_P_GA_SBX = 5,
+    OR_P_GA_PM = 6, OR_P_MIG = 7, OR_P_XFER = 8, OR_P_ISLX = 9
+};
+
+/* One continuous task.  z = Q (x - o) (ROTATE), z = x - o (SHIFT),
+ * z = M * (x - o) (DIAG, reference transform z = M (x + b) with o = -b). */
+typedef struct {
+    int32_t func, dim, pop, xform;
+    float lo, hi;
+    const float* rot;    /* dim*dim row-major Q[j][d]   (ROTATE) */
+    const float* shift;  /* dim                          (all)    */
+    const float* diag;   /* dim                          (DIAG)   */
+} or_task;
+
+typedef struct {
+    int32_t algo;
+    float F, CR;                  /* DE */
+    float eta_c, eta_m, pm;       /* GA: SBX index, polynomial-mutation index and rate */
+    int32_t n_islands;            /* sub-populations per task (mates drawn inside one) */
+    int32_t exchange_interval;    /* cross-island resampling period (0 = off) */
+    int32_t migrate_interval;     /* intra-task island migration period (0 = off) */
+    int32_t migrate_count;
+    int32_t transfer_interval;    /* inter-task transfer period (0 = off) */
+    int32_t transfer_count;
+    float transfer_cr;            /* keep own gene if u <= transfer_cr (0 = take all) */
+    uint64_t seed;
+} or_params;
+
+/* ---- Philox ------------------------------------------------------------ */
+void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void or_draw_block(uint64_t seed, uint32_t task, uint32_t gen, uint32_t purpose,
+                   uint32_t index, uint32_t sub, uint32_t block, uint32_t out[4]);
+
+/* ---- FP32 function definitions (bit-exact contract) ---------------------- */
+float or_sinpi_sq(float v);      /* sin^2(pi v) */
+float or_sin2pi(float v);         /* sin(2 pi v) */
+float or_expm1(float x);          /* e^x - 1, x <= 88 */
+float or_exp2(float y);
+float or_log2(float x);
+float or_pow(float x, float y);   /* x >= 0 */
+float or_weier_term(float z);     /* sum_k 2^(1-k) sin^2(pi 3^k z), k = 0..20 */
+
+/* Per-dimension coefficient tables (Elliptic weights / Griewank scales). */
+void or_func_table(int32_t func, int32_t dim, float* table);
+
+/* Objective from z (contract arithmetic), table from or_func_table. */
+float or_finish32(const or_task* t, const float* z, const float* tab);
+/* FP32 evaluation of n points x[n][dim] (contract arithmetic). */
+void or_eval32(const or_task* t, int32_t n, const float* x, float* f);
+/* FP64 textbook evaluation (independent formulas, libm). */
+void or_eval64(const or_task* t, int32_t n, const double* x, double* f);
+/* FP64 gradient of the textbook objective at x[n][dim]. */
+void or_grad64(const or_task* t, int32_t n, const double* x, double* g);
+
+/* ---- Engine -------------------------------------------------------------- */
+typedef struct or_engine or_engine;
+or_engine* or_create(const or_task* tasks, int32_t n_tasks, const or_params* p);
+void or_destroy(or_engine* e);
+/* Advance every task by n generations.  n_threads <= 0: OpenMP default. */
+void or_run(or_engine* e, int32_t n_gens, int32_t n_threads);
+int32_t or_generation(const or_engine* e);
+void or_get_population(const or_engine* e, int32_t task, float* x, float* fit);
+void or_set_population(or_engine* e, int32_t task, const float* x, const float* fit);
+void or_get_best(const or_engine* e, float* fit, float* x_best, int32_t x_stride);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

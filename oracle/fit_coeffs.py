@@ -1,0 +1,48 @@
+"""Derives the float32 polynomial coefficients used by the FP32 evaluator.
+
+Test infrastructure / documentation: prints Chebyshev (near-minimax) fits,
+computed in 60-digit arithmetic with mpmath, rounded to float32, together with
+the maximum relative error of the rounded polynomial measured in float64.
+Both the device code and the CPU oracle carry these constants; the oracle's
+unit tests check the resulting functions against float64 references.
+"""
+import mpmath as mp
+import numpy as np
+
+mp.mp.dps = 60
+
+
+def fit(name, f, a, b, n, scale=None):
+    poly, err = mp.chebyfit(f, [a, b], n, error=True)
+    # mpmath returns highest-degree first
+    c32 = [np.float32(float(c)) for c in poly]
+    xs = np.linspace(a, b, 20001)
+    ref = np.array([float(f(mp.mpf(float(x)))) for x in xs])
+    approx = np.zeros_like(xs)
+    for c in c32:
+        approx = approx * xs + np.float64(c)
+    rel = np.max(np.abs(approx - ref) / np.maximum(np.abs(ref), 1e-300))
+    print(f"{name}: degree {n - 1} on [{a}, {b}], max rel err (float32 coeffs) {rel:.3e}")
+    print("  " + ", ".join(f"{float(c):.9e}f" for c in c32))
+    return c32
+
+
+def main():
+    pi = mp.pi
+    # sin^2(pi*b) = x * S2(x), x = b^2, b in [0, 1/4]
+    fit("S2", lambda x: (mp.sin(pi * mp.sqrt(x)) ** 2 / x) if x > 0 else pi ** 2, 0.0, 1.0 / 16, 7)
+    # sin(2*pi*b) = b * SN(x), x = b^2, b in [0, 1/4]
+    fit("SN", lambda x: (mp.sin(2 * pi * mp.sqrt(x)) / mp.sqrt(x)) if x > 0 else 2 * pi, 0.0, 1.0 / 16, 7)
+    # expm1(r) = r + r^2 * E(r), r in [-ln2/2, ln2/2]
+    h = float(mp.log(2) / 2)
+    fit("E", lambda r: ((mp.expm1(r) - r) / r ** 2) if r != 0 else mp.mpf(1) / 2, -h, h, 7)
+    # 2^f, f in [-1/2, 1/2]
+    fit("X2", lambda t: mp.power(2, t), -0.5, 0.5, 8)
+    # log2((1+s)/(1-s)) = s * L(s^2), s in [0, 0.1716] -> x = s^2 in [0, 0.02944]
+    smax = float((mp.sqrt(2) - 1) / (mp.sqrt(2) + 1))
+    fit("L", lambda x: (mp.log((1 + mp.sqrt(x)) / (1 - mp.sqrt(x)), 2) / mp.sqrt(x)) if x > 0 else 2 / mp.log(2),
+        0.0, smax ** 2, 6)
+
+
+if __name__ == "__main__":
+    main()

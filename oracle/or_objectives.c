@@ -1,0 +1,302 @@
+/*
+ * or_objectives.c -- the eight continuous objectives of the oracle.
+ * TEST INFRASTRUCTURE ONLY (see emtf_oracle.h).
+ *
+ * Follows the reference evaluator `evaluation`, proposed model/SG-EMTF_DE.cu:834-1069
+ * (GA copy SG-EMTF.cu:898-1111), with its transform z_d = M_d (x_d + b_d)
+ * (SG-EMTF_DE.cu:856-1054, written here as M (x - o) with o = -b) and the
+ * rotated transform z = Q (x - o) of the benchmark configurations.
+ * Function ids 0..6 are the reference's case labels (Sphere, Rosenbrock,
+ * Ackley, Rastrigin, Griewank, Weierstrass, Schwefel); 7 is the CEC
+ * high-conditioned Elliptic, which the reference does not have.
+ *
+ * FP32 forms are the cancellation-free restatements of the same functions:
+ *   Rastrigin   z^2 - 10 cos(2 pi z) + 10   = z^2 + 20 sin^2(pi z)
+ *   Ackley      20 + e - 20 e^{-0.2 r} - e^{C/D} = -20 expm1(-0.2 r) - e expm1(-2 S/D),
+ *               r = sqrt(sum z^2 / D), S = sum sin^2(pi z)
+ *   Griewank    1 - prod cos(z_j / sqrt(j+1)) = q_D with q_{k+1} = q_k + a_k (1 - q_k),
+ *               a_k = 2 sin^2(z_k / (2 sqrt(k+1)))
+ *   Weierstrass sum_k 0.5^k [cos(2 pi 3^k (z + 1/2)) - cos(pi 3^k)] = sum_k 2^{1-k} sin^2(pi 3^k z)
+ * so that each equals zero at its optimum.  The FP64 evaluator below uses
+ * the textbook forms, which makes it an independent check of the rewrite.
+ */
+#include "emtf_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SCHWEFEL_C 418.9828872724338
+#define SCHWEFEL_CF 418.982887f
+#define E_F 2.71828175f
+
+void or_func_table(int32_t func, int32_t dim, float* table) {
+    for (int j = 0; j < dim; ++j) {
+        double v = 0.0;
+        if (func == OR_FUNC_ELLIPTIC) v = (dim > 1) ? pow(1.0e6, (double)j / (double)(dim - 1)) : 1.0;
+        else if (func == OR_FUNC_GRIEWANK) v = 1.0 / (2.0 * M_PI * sqrt((double)(j + 1)));
+        table[j] = (float)v;
+    }
+}
+
+/* z from x under the task transform (contract arithmetic). */
+static void transform32(const or_task* t, const float* x, float* w, float* z) {
+    const int D = t->dim;
+    for (int d = 0; d < D; ++d) w[d] = x[d] - t->shift[d];
+    if (t->xform == OR_XFORM_SHIFT) {
+        for (int j = 0; j < D; ++j) z[j] = w[j];
+    } else if (t->xform == OR_XFORM_DIAG) {
+        for (int j = 0; j < D; ++j) z[j] = t->diag[j] * w[j];
+    } else {
+        const int Dp = (D + 1) & ~1;
+        for (int j = 0; j < D; ++j) {
+            float acc = 0.0f;
+            const float* q = t->rot + (size_t)j * D;
+            for (int d = 0; d < D; ++d) acc = fmaf(q[d], w[d], acc);
+            for (int d = D; d < Dp; ++d) acc = fmaf(0.0f, 0.0f, acc);
+            z[j] = acc;
+        }
+    }
+}
+
+#define ACC_A(j) ((((j) >> 2) & 1) == 0)
+
+float or_finish32(const or_task* t, const float* z, const float* tab) {
+    const int D = t->dim;
+    float A = 0.0f, B = 0.0f, A2 = 0.0f, B2 = 0.0f;
+    switch (t->func) {
+    case OR_FUNC_SPHERE:
+        for (int j = 0; j < D; ++j) { float v = z[j] * z[j]; if (ACC_A(j)) A = A + v; else B = B + v; }
+        return A + B;
+    case OR_FUNC_ROSENBROCK:
+        for (int j = 0; j + 1 < D; ++j) {
+            float zz = z[j] * z[j];
+            float a = zz - z[j + 1];
+            float b = z[j] - 1.0f;
+            float t1 = 100.0f * a;
+            t1 = t1 * a;
+            float b2 = b * b;
+            float v = t1 + b2;
+            if (ACC_A(j)) A = A + v; else B = B + v;
+        }
+        return A + B;
+    case OR_FUNC_ACKLEY: {
+        for (int j = 0; j < D; ++j) {
+            float v = z[j] * z[j];
+            float s = or_sinpi_sq(z[j]);
+            if (ACC_A(j)) { A = A + v; A2 = A2 + s; } else { B = B + v; B2 = B2 + s; }
+        }
+        float Df = (float)D;
+        float S = A + B, C = A2 + B2;
+        float q = S / Df;
+        float r = sqrtf(q);
+        float a = -0.2f * r;
+        float e1 = or_expm1(a);
+        float t1 = -20.0f * e1;
+        float c2 = C / Df;
+        float b = -2.0f * c2;
+        float e2 = or_expm1(b);
+        float t2 = E_F * e2;
+        return t1 - t2;
+    }
+    case OR_FUNC_RASTRIGIN:
+        for (int j = 0; j < D; ++j) {
+            float zz = z[j] * z[j];
+            float s = or_sinpi_sq(z[j]);
+            float s20 = 20.0f * s;
+            float v = zz + s20;
+            if (ACC_A(j)) A = A + v; else B = B + v;
+        }
+        return A + B;
+    case OR_FUNC_GRIEWANK: {
+        float qA = 0.0f, qB = 0.0f;
+        for (int j = 0; j < D; ++j) {
+            float v = z[j] * z[j];
+            float tt = z[j] * tab[j];
+            float s = or_sinpi_sq(tt);
+            float a = 2.0f * s;
+            if (ACC_A(j)) { A = A + v; float om = 1.0f - qA; float aq = a * om; qA = qA + aq; }
+            else { B = B + v; float om = 1.0f - qB; float aq = a * om; qB = qB + aq; }
+        }
+        float om = 1.0f - qA;
+        float tq = qB * om;
+        float Q = qA + tq;
+        float S = A + B;
+        float s4 = S / 4000.0f;
+        return s4 + Q;
+    }
+    case OR_FUNC_WEIERSTRASS:
+        for (int j = 0; j < D; ++j) { float v = or_weier_term(z[j]); if (ACC_A(j)) A = A + v; else B = B + v; }
+        return A + B;
+    case OR_FUNC_SCHWEFEL: {
+        for (int j = 0; j < D; ++j) {
+            float sq = sqrtf(fabsf(z[j]));
+            float tv = sq * 0.159154937f;
+            float s = or_sin2pi(tv);
+            float v = z[j] * s;
+            if (ACC_A(j)) A = A + v; else B = B + v;
+        }
+        float cd = SCHWEFEL_CF * (float)D;
+        float S = A + B;
+        return cd - S;
+    }
+    case OR_FUNC_ELLIPTIC:
+        for (int j = 0; j < D; ++j) {
+            float zz = z[j] * z[j];
+            float v = tab[j] * zz;
+            if (ACC_A(j)) A = A + v; else B = B + v;
+        }
+        return A + B;
+    default:
+        return NAN;
+    }
+}
+
+void or_eval32(const or_task* t, int32_t n, const float* x, float* f) {
+    const int D = t->dim;
+    float* w = (float*)malloc(sizeof(float) * (size_t)D * 3);
+    float* z = w + D;
+    float* tab = z + D;
+    or_func_table(t->func, D, tab);
+    for (int i = 0; i < n; ++i) {
+        transform32(t, x + (size_t)i * D, w, z);
+        f[i] = or_finish32(t, z, tab);
+    }
+    free(w);
+}
+
+/* ---- FP64 textbook forms ----------------------------------------------- */
+static void transform64(const or_task* t, const double* x, double* z) {
+    const int D = t->dim;
+    for (int j = 0; j < D; ++j) {
+        if (t->xform == OR_XFORM_ROTATE) {
+            double acc = 0.0;
+            for (int d = 0; d < D; ++d) acc += (double)t->rot[(size_t)j * D + d] * (x[d] - (double)t->shift[d]);
+            z[j] = acc;
+        } else if (t->xform == OR_XFORM_DIAG) {
+            z[j] = (double)t->diag[j] * (x[j] - (double)t->shift[j]);
+        } else {
+            z[j] = x[j] - (double)t->shift[j];
+        }
+    }
+}
+
+static double func64(int func, int D, const double* z) {
+    double s = 0.0, c = 0.0, p = 1.0;
+    switch (func) {
+    case OR_FUNC_SPHERE:
+        for (int j = 0; j < D; ++j) s += z[j] * z[j];
+        return s;
+    case OR_FUNC_ROSENBROCK:
+        for (int j = 0; j + 1 < D; ++j) {
+            double a = z[j] * z[j] - z[j + 1], b = z[j] - 1.0;
+            s += 100.0 * a * a + b * b;
+        }
+        return s;
+    case OR_FUNC_ACKLEY:
+        for (int j = 0; j < D; ++j) { s += z[j] * z[j]; c += cos(2.0 * M_PI * z[j]); }
+        return -20.0 * exp(-0.2 * sqrt(s / D)) - exp(c / D) + 20.0 + M_E;
+    case OR_FUNC_RASTRIGIN:
+        for (int j = 0; j < D; ++j) s += z[j] * z[j] - 10.0 * cos(2.0 * M_PI * z[j]) + 10.0;
+        return s;
+    case OR_FUNC_GRIEWANK:
+        for (int j = 0; j < D; ++j) { s += z[j] * z[j]; p *= cos(z[j] / sqrt((double)(j + 1))); }
+        return 1.0 + s / 4000.0 - p;
+    case OR_FUNC_WEIERSTRASS: {
+        double w0 = 0.0;
+        for (int k = 0; k <= 20; ++k) w0 += pow(0.5, k) * cos(M_PI * pow(3.0, k));
+        for (int j = 0; j < D; ++j)
+            for (int k = 0; k <= 20; ++k) s += pow(0.5, k) * cos(2.0 * M_PI * pow(3.0, k) * (z[j] + 0.5));
+        return s - D * w0;
+    }
+    case OR_FUNC_SCHWEFEL:
+        for (int j = 0; j < D; ++j) s += z[j] * sin(sqrt(fabs(z[j])));
+        return SCHWEFEL_C * D - s;
+    case OR_FUNC_ELLIPTIC:
+        for (int j = 0; j < D; ++j) s += ((D > 1) ? pow(1.0e6, (double)j / (D - 1)) : 1.0) * z[j] * z[j];
+        return s;
+    }
+    return NAN;
+}
+
+void or_eval64(const or_task* t, int32_t n, const double* x, double* f) {
+    double* z = (double*)malloc(sizeof(double) * (size_t)t->dim);
+    for (int i = 0; i < n; ++i) {
+        transform64(t, x + (size_t)i * t->dim, z);
+        f[i] = func64(t->func, t->dim, z);
+    }
+    free(z);
+}
+
+/* d f / d z (textbook) */
+static void dfdz64(int func, int D, const double* z, double* g) {
+    double s = 0.0, c = 0.0;
+    for (int j = 0; j < D; ++j) g[j] = 0.0;
+    switch (func) {
+    case OR_FUNC_SPHERE:
+        for (int j = 0; j < D; ++j) g[j] = 2.0 * z[j];
+        break;
+    case OR_FUNC_ROSENBROCK:
+        for (int j = 0; j + 1 < D; ++j) {
+            double a = z[j] * z[j] - z[j + 1];
+            g[j] += 400.0 * a * z[j] + 2.0 * (z[j] - 1.0);
+            g[j + 1] += -200.0 * a;
+        }
+        break;
+    case OR_FUNC_ACKLEY: {
+        for (int j = 0; j < D; ++j) { s += z[j] * z[j]; c += cos(2.0 * M_PI * z[j]); }
+        double r = sqrt(s / D);
+        double k1 = (r > 0.0) ? 2.0 * exp(-0.2 * r) / (r * D) : 0.0;
+        double k2 = exp(c / D) * 2.0 * M_PI / D;
+        for (int j = 0; j < D; ++j) g[j] = k1 * z[j] + k2 * sin(2.0 * M_PI * z[j]);
+        break;
+    }
+    case OR_FUNC_RASTRIGIN:
+        for (int j = 0; j < D; ++j) g[j] = 2.0 * z[j] + 20.0 * M_PI * sin(2.0 * M_PI * z[j]);
+        break;
+    case OR_FUNC_GRIEWANK:
+        for (int j = 0; j < D; ++j) {
+            double pr = 1.0;
+            for (int k = 0; k < D; ++k) if (k != j) pr *= cos(z[k] / sqrt((double)(k + 1)));
+            double sj = sqrt((double)(j + 1));
+            g[j] = z[j] / 2000.0 + pr * sin(z[j] / sj) / sj;
+        }
+        break;
+    case OR_FUNC_WEIERSTRASS:
+        for (int j = 0; j < D; ++j)
+            for (int k = 0; k <= 20; ++k)
+                g[j] -= pow(0.5, k) * 2.0 * M_PI * pow(3.0, k) * sin(2.0 * M_PI * pow(3.0, k) * (z[j] + 0.5));
+        break;
+    case OR_FUNC_SCHWEFEL:
+        for (int j = 0; j < D; ++j) {
+            double sq = sqrt(fabs(z[j]));
+            g[j] = -(sin(sq) + 0.5 * sq * cos(sq));
+        }
+        break;
+    case OR_FUNC_ELLIPTIC:
+        for (int j = 0; j < D; ++j) g[j] = 2.0 * ((D > 1) ? pow(1.0e6, (double)j / (D - 1)) : 1.0) * z[j];
+        break;
+    }
+}
+
+void or_grad64(const or_task* t, int32_t n, const double* x, double* g) {
+    const int D = t->dim;
+    double* z = (double*)malloc(sizeof(double) * (size_t)D * 2);
+    double* gz = z + D;
+    for (int i = 0; i < n; ++i) {
+        transform64(t, x + (size_t)i * D, z);
+        dfdz64(t->func, D, z, gz);
+        double* gx = g + (size_t)i * D;
+        for (int d = 0; d < D; ++d) {
+            if (t->xform == OR_XFORM_ROTATE) {
+                double acc = 0.0;
+                for (int j = 0; j < D; ++j) acc += (double)t->rot[(size_t)j * D + d] * gz[j];
+                gx[d] = acc;
+            } else if (t->xform == OR_XFORM_DIAG) {
+                gx[d] = (double)t->diag[d] * gz[d];
+            } else {
+                gx[d] = gz[d];
+            }
+        }
+    }
+    free(z);
+}

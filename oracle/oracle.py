@@ -1,0 +1,205 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py; the product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+FUNC_NAMES = ["sphere", "rosenbrock", "ackley", "rastrigin", "griewank", "weierstrass", "schwefel", "elliptic"]
+XFORM_SHIFT, XFORM_ROTATE, XFORM_DIAG = 0, 1, 2
+ALGO_DE_RAND_1_BIN, ALGO_DE_CURRENT_1_BIN, ALGO_GA_SBX_PM = 0, 1, 2
+
+
+class OrTask(ctypes.Structure):
+    _fields_ = [("func", ctypes.c_int32), ("dim", ctypes.c_int32), ("pop", ctypes.c_int32),
+                ("xform", ctypes.c_int32), ("lo", ctypes.c_float), ("hi", ctypes.c_float),
+                ("rot", ctypes.POINTER(ctypes.c_float)), ("shift", ctypes.POINTER(ctypes.c_float)),
+                ("diag", ctypes.POINTER(ctypes.c_float))]
+
+
+class OrParams(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_int32), ("F", ctypes.c_float), ("CR", ctypes.c_float),
+                ("eta_c", ctypes.c_float), ("eta_m", ctypes.c_float), ("pm", ctypes.c_float),
+                ("n_islands", ctypes.c_int32), ("exchange_interval", ctypes.c_int32),
+                ("migrate_interval", ctypes.c_int32), ("migrate_count", ctypes.c_int32),
+                ("transfer_interval", ctypes.c_int32), ("transfer_count", ctypes.c_int32),
+                ("transfer_cr", ctypes.c_float), ("seed", ctypes.c_uint64)]
+
+
+def build(force=False):
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        f32p = ctypes.POINTER(ctypes.c_float)
+        f64p = ctypes.POINTER(ctypes.c_double)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.or_philox.argtypes = [u32p, u32p, u32p]
+        L.or_draw_block.argtypes = [ctypes.c_uint64] + [ctypes.c_uint32] * 6 + [u32p]
+        for n in ["or_sinpi_sq", "or_sin2pi", "or_expm1", "or_exp2", "or_log2", "or_weier_term"]:
+            getattr(L, n).argtypes = [ctypes.c_float]
+            getattr(L, n).restype = ctypes.c_float
+        L.or_pow.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.or_pow.restype = ctypes.c_float
+        L.or_func_table.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p]
+        L.or_eval32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
+        L.or_eval64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
+        L.or_grad64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
+        L.or_create.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, ctypes.POINTER(OrParams)]
+        L.or_create.restype = ctypes.c_void_p
+        L.or_destroy.argtypes = [ctypes.c_void_p]
+        L.or_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+        L.or_generation.argtypes = [ctypes.c_void_p]
+        L.or_generation.restype = ctypes.c_int32
+        L.or_get_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
+        L.or_set_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
+        L.or_get_best.argtypes = [ctypes.c_void_p, f32p, f32p, ctypes.c_int32]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().or_philox(c, k, o)
+    return list(o)
+
+
+def draw_block(seed, task, gen, purpose, index, sub, block):
+    o = (ctypes.c_uint32 * 4)()
+    lib().or_draw_block(seed, task, gen, purpose, index, sub, block, o)
+    return list(o)
+
+
+def scalar(name, *args):
+    return getattr(lib(), "or_" + name)(*args)
+
+
+def func_table(func, dim):
+    t = np.zeros(dim, np.float32)
+    lib().or_func_table(func, dim, _fp(t))
+    return t
+
+
+class _TaskHolder:
+    """Keeps the numpy arrays of one task alive while C holds pointers."""
+
+    def __init__(self, task):
+        self.func = int(task["func"])
+        self.dim = int(task["dim"])
+        self.pop = int(task.get("pop", 32))
+        self.xform = int(task.get("xform", XFORM_SHIFT))
+        self.lo = float(task.get("lo", -100.0))
+        self.hi = float(task.get("hi", 100.0))
+        D = self.dim
+        self.shift = np.ascontiguousarray(task.get("shift", np.zeros(D)), dtype=np.float32)
+        self.rot = np.ascontiguousarray(task["rot"], dtype=np.float32) if task.get("rot") is not None else None
+        self.diag = np.ascontiguousarray(task["diag"], dtype=np.float32) if task.get("diag") is not None else None
+        self.c = OrTask(self.func, D, self.pop, self.xform, self.lo, self.hi,
+                        _fp(self.rot) if self.rot is not None else None, _fp(self.shift),
+                        _fp(self.diag) if self.diag is not None else None)
+
+
+def eval32(task, x):
+    h = _TaskHolder(task)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, h.dim)
+    f = np.zeros(x.shape[0], np.float32)
+    lib().or_eval32(ctypes.byref(h.c), x.shape[0], _fp(x), _fp(f))
+    return f
+
+
+def eval64(task, x):
+    h = _TaskHolder(task)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, h.dim)
+    f = np.zeros(x.shape[0], np.float64)
+    lib().or_eval64(ctypes.byref(h.c), x.shape[0], _dp(x), _dp(f))
+    return f
+
+
+def grad64(task, x):
+    h = _TaskHolder(task)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, h.dim)
+    g = np.zeros_like(x)
+    lib().or_grad64(ctypes.byref(h.c), x.shape[0], _dp(x), _dp(g))
+    return g
+
+
+def make_params(**kw):
+    d = dict(algo=ALGO_DE_RAND_1_BIN, F=0.5, CR=0.9, eta_c=2.0, eta_m=5.0, pm=1.0, n_islands=1,
+             exchange_interval=0, migrate_interval=0, migrate_count=0, transfer_interval=0,
+             transfer_count=0, transfer_cr=0.0, seed=0)
+    d.update(kw)
+    return OrParams(**d)
+
+
+class Engine:
+    """The oracle's multitasking engine over a list of task dicts."""
+
+    def __init__(self, tasks, **params):
+        self._holders = [_TaskHolder(t) for t in tasks]
+        arr = (OrTask * len(tasks))(*[h.c for h in self._holders])
+        self._p = make_params(**params)
+        self._h = lib().or_create(arr, len(tasks), ctypes.byref(self._p))
+        self.n_tasks = len(tasks)
+
+    def run(self, gens, threads=0):
+        lib().or_run(self._h, int(gens), int(threads))
+
+    @property
+    def generation(self):
+        return lib().or_generation(self._h)
+
+    def population(self, t):
+        h = self._holders[t]
+        x = np.zeros((h.pop, h.dim), np.float32)
+        f = np.zeros(h.pop, np.float32)
+        lib().or_get_population(self._h, t, _fp(x), _fp(f))
+        return x, f
+
+    def set_population(self, t, x, f):
+        x = np.ascontiguousarray(x, np.float32)
+        f = np.ascontiguousarray(f, np.float32)
+        lib().or_set_population(self._h, t, _fp(x), _fp(f))
+
+    def best(self):
+        dmax = max(h.dim for h in self._holders)
+        f = np.zeros(self.n_tasks, np.float32)
+        x = np.zeros((self.n_tasks, dmax), np.float32)
+        lib().or_get_best(self._h, _fp(f), _fp(x), dmax)
+        return f, x
+
+    def close(self):
+        if self._h:
+            lib().or_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,55 @@
+"""Builds libemtf.so (HIP kernels + C ABI) in-tree for gfx950.
+
+island.hip is compiled once per objective id (-DEMTF_FUNC=0..7) so every
+kernel carries only its own epilogue; the objects are linked with the host
+runtime into sg-emtf_amd/libemtf.so.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libemtf.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -ffp-contract=off: the arithmetic contract forbids implicit fma contraction.
+# -packed-fp32-ops: keep the device compiler away from v_pk_{add,mul,fma}_f32.
+COMMON = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off", "-fPIC",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+N_FUNCS = 8
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = [l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l and l.strip()]
+    return r.returncode, "\n".join(msgs)
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    units = [([HIPCC] + COMMON + [f"-DEMTF_FUNC={f}", "-c", "-o", os.path.join(OBJ, f"island_f{f}.o"),
+                                  os.path.join(CSRC, "island.hip")]) for f in range(N_FUNCS)]
+    units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, "emtf_host.o"), os.path.join(CSRC, "emtf_host.cpp")])
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(_run, units))
+    for (rc, msg), cmd in zip(results, units):
+        if rc != 0:
+            sys.stderr.write(msg + "\n")
+            raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+        if verbose and msg:
+            print(msg)
+    objs = [os.path.join(OBJ, f"island_f{f}.o") for f in range(N_FUNCS)] + [os.path.join(OBJ, "emtf_host.o")]
+    rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
+    if rc != 0:
+        sys.stderr.write(msg + "\n")
+        raise RuntimeError("link of libemtf.so failed")
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,353 @@
+// emtf_host.cpp -- host runtime behind include/emtf.h.
+//
+// Owns the device state of a batch of tasks (one contiguous arena per array,
+// per-task offsets in a descriptor table) and schedules epochs of the island
+// kernel.  An epoch never crosses a transfer boundary, so the kernel boundary
+// is the synchronisation point of the elite mailbox: the launch that ends at
+// generation g writes mailbox buffer (g / interval) & 1, the next launch reads
+// it while writing the other one.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/emtf.h"
+#include "emtf_types.h"
+
+// One launcher per objective (island.hip built once per objective id).
+#define EMTF_DECL_LAUNCH(f)                                                                               \
+    extern "C" int emtf_launch_epoch_f##f(const emtf::EpochArgs* args, const int* group, int n, int nt, \
+                                          int lds_bytes, hipStream_t stream);
+EMTF_DECL_LAUNCH(0) EMTF_DECL_LAUNCH(1) EMTF_DECL_LAUNCH(2) EMTF_DECL_LAUNCH(3)
+EMTF_DECL_LAUNCH(4) EMTF_DECL_LAUNCH(5) EMTF_DECL_LAUNCH(6) EMTF_DECL_LAUNCH(7)
+extern "C" int emtf_lds_floats(int D, int pop, int nt);
+
+typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, int, hipStream_t);
+static const launch_fn kLaunch[8] = {emtf_launch_epoch_f0, emtf_launch_epoch_f1, emtf_launch_epoch_f2,
+                                     emtf_launch_epoch_f3, emtf_launch_epoch_f4, emtf_launch_epoch_f5,
+                                     emtf_launch_epoch_f6, emtf_launch_epoch_f7};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OR_FAIL(expr)                                                                            \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return fail(EMTF_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+}  // namespace
+
+struct emtf_handle {
+    int T = 0;
+    emtf::EngineParams p{};
+    std::vector<emtf::TaskDev> tasks;
+    emtf::TaskDev* d_tasks = nullptr;
+    float* d_X = nullptr;
+    float* d_fit = nullptr;
+    float* d_Q = nullptr;
+    float* d_shift = nullptr;
+    float* d_tab = nullptr;
+    float* d_mailbox = nullptr;
+    float* d_scratch = nullptr;
+    int* d_groups = nullptr;     // task ids grouped by objective
+    int group_off[9] = {0};
+    int group_lds[8] = {0};
+    int64_t mailbox_stride = 0;
+    int nt = 1;
+    int gen = 0;
+    int max_dim = 0;
+    hipStream_t stream = nullptr;
+};
+
+static void free_handle(emtf_handle* h) {
+    if (!h) return;
+    (void)hipFree(h->d_tasks);
+    (void)hipFree(h->d_X);
+    (void)hipFree(h->d_fit);
+    (void)hipFree(h->d_Q);
+    (void)hipFree(h->d_shift);
+    (void)hipFree(h->d_tab);
+    (void)hipFree(h->d_mailbox);
+    (void)hipFree(h->d_scratch);
+    (void)hipFree(h->d_groups);
+    delete h;
+}
+
+// Per-dimension table: Elliptic weights 10^(6 j/(D-1)), Griewank 1/(2 pi sqrt(j+1)).
+static void func_table(int func, int dim, float* t) {
+    for (int j = 0; j < dim; ++j) {
+        double v = 0.0;
+        if (func == EMTF_ELLIPTIC) v = (dim > 1) ? std::pow(1.0e6, (double)j / (double)(dim - 1)) : 1.0;
+        else if (func == EMTF_GRIEWANK) v = 1.0 / (2.0 * M_PI * std::sqrt((double)(j + 1)));
+        t[j] = (float)v;
+    }
+}
+
+static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) {
+    emtf::EpochArgs a{};
+    a.tasks = h->d_tasks;
+    a.n_tasks = h->T;
+    a.g0 = g0;
+    a.g1 = g1;
+    a.init = init;
+    a.X = h->d_X;
+    a.fit = h->d_fit;
+    a.Q = h->d_Q;
+    a.shift = h->d_shift;
+    a.tab = h->d_tab;
+    a.mailbox = h->d_mailbox;
+    a.mailbox_stride = h->mailbox_stride;
+    a.scratch = h->d_scratch;
+    a.p = h->p;
+    for (int f = 0; f < 8; ++f) {
+        const int n = h->group_off[f + 1] - h->group_off[f];
+        if (n == 0) continue;
+        const int rc = kLaunch[f](&a, h->d_groups + h->group_off[f], n, h->nt, h->group_lds[f], stream);
+        if (rc != 0)
+            return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
+    }
+    return EMTF_OK;
+}
+
+extern "C" {
+
+const char* emtf_last_error(void) { return g_err.c_str(); }
+
+int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params* prm, emtf_handle** out) {
+    g_err.clear();
+    if (!out) return fail(EMTF_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!tasks || !prm || n_tasks < 1) return fail(EMTF_E_INVALID, "tasks/params missing or n_tasks < 1");
+    if (prm->algo < EMTF_DE_RAND_1_BIN || prm->algo > EMTF_GA_SBX_PM) return fail(EMTF_E_INVALID, "unknown algo");
+    const int nis = prm->n_islands < 1 ? 1 : prm->n_islands;
+    if (prm->transfer_count < 0 || prm->migrate_count < 0 || prm->transfer_interval < 0 ||
+        prm->migrate_interval < 0 || prm->exchange_interval < 0)
+        return fail(EMTF_E_INVALID, "negative interval or count");
+    if (!(prm->F == prm->F) || !(prm->CR == prm->CR)) return fail(EMTF_E_INVALID, "F/CR is NaN");
+
+    int max_dim = 0;
+    for (int t = 0; t < n_tasks; ++t) {
+        const emtf_task_desc& d = tasks[t];
+        char buf[160];
+        if (d.func < 0 || d.func > EMTF_ELLIPTIC) { snprintf(buf, sizeof buf, "task %d: unknown func %d", t, d.func); return fail(EMTF_E_INVALID, buf); }
+        if (d.dim < 1) { snprintf(buf, sizeof buf, "task %d: dim %d < 1", t, d.dim); return fail(EMTF_E_INVALID, buf); }
+        if (d.dim > 128) { snprintf(buf, sizeof buf, "task %d: dim %d > 128 not built", t, d.dim); return fail(EMTF_E_UNSUPPORTED, buf); }
+        if (d.pop < 2 || d.pop % nis != 0) { snprintf(buf, sizeof buf, "task %d: pop %d not a multiple of n_islands %d", t, d.pop, nis); return fail(EMTF_E_INVALID, buf); }
+        if (d.pop > 1024) { snprintf(buf, sizeof buf, "task %d: pop %d > 1024", t, d.pop); return fail(EMTF_E_UNSUPPORTED, buf); }
+        if (!(d.lo < d.hi) || !std::isfinite(d.lo) || !std::isfinite(d.hi)) { snprintf(buf, sizeof buf, "task %d: bad box", t); return fail(EMTF_E_INVALID, buf); }
+        if (d.xform < EMTF_SHIFT || d.xform > EMTF_DIAG) { snprintf(buf, sizeof buf, "task %d: unknown xform", t); return fail(EMTF_E_INVALID, buf); }
+        if (d.xform == EMTF_ROTATE && !d.rot) { snprintf(buf, sizeof buf, "task %d: rotation matrix missing", t); return fail(EMTF_E_INVALID, buf); }
+        if (d.xform == EMTF_DIAG && !d.diag) { snprintf(buf, sizeof buf, "task %d: diagonal missing", t); return fail(EMTF_E_INVALID, buf); }
+        const int m = d.pop / nis;
+        if (prm->algo == EMTF_GA_SBX_PM && (m % 2)) { snprintf(buf, sizeof buf, "task %d: GA needs an even island size", t); return fail(EMTF_E_INVALID, buf); }
+        if (prm->transfer_interval > 0 && 2 * prm->transfer_count > d.pop) { snprintf(buf, sizeof buf, "task %d: transfer_count > pop/2", t); return fail(EMTF_E_INVALID, buf); }
+        if (nis > 1 && prm->migrate_interval > 0 && 2 * prm->migrate_count > m) { snprintf(buf, sizeof buf, "task %d: migrate_count > island/2", t); return fail(EMTF_E_INVALID, buf); }
+        if (d.dim > max_dim) max_dim = d.dim;
+    }
+
+    emtf_handle* h = new emtf_handle();
+    h->T = n_tasks;
+    h->max_dim = max_dim;
+    h->nt = max_dim <= 32 ? 1 : (max_dim <= 64 ? 2 : 4);
+    emtf::EngineParams& p = h->p;
+    p.algo = prm->algo;
+    p.F = prm->F;
+    p.CR = prm->CR;
+    p.eta_c = prm->eta_c;
+    p.eta_m = prm->eta_m;
+    p.pm = prm->pm;
+    p.n_islands = nis;
+    p.exchange_interval = prm->exchange_interval;
+    p.migrate_interval = prm->migrate_interval;
+    p.migrate_count = prm->migrate_count;
+    p.transfer_interval = prm->transfer_interval;
+    p.transfer_count = prm->transfer_interval > 0 ? prm->transfer_count : 0;
+    p.transfer_cr = prm->transfer_cr;
+    p.seed_lo = (uint32_t)prm->seed;
+    p.seed_hi = (uint32_t)(prm->seed >> 32);
+
+    int64_t nX = 0, nF = 0, nQ = 0, nV = 0, nM = 0, nS = 0;
+    int lds = 0;
+    h->tasks.resize(n_tasks);
+    for (int t = 0; t < n_tasks; ++t) {
+        const emtf_task_desc& d = tasks[t];
+        emtf::TaskDev& td = h->tasks[t];
+        std::memset(&td, 0, sizeof td);
+        td.func = d.func;
+        td.dim = d.dim;
+        td.pop = d.pop;
+        td.xform = d.xform;
+        td.lo = d.lo;
+        td.hi = d.hi;
+        td.x_off = nX;
+        td.fit_off = nF;
+        td.q_off = nQ;
+        td.v_off = nV;
+        td.mb_off = nM;
+        td.scr_off = nS;
+        const int Dh = (d.dim + 1) >> 1, DhP = (Dh + 3) & ~3, RS = 2 * DhP + 4;
+        nX += (int64_t)d.pop * d.dim;
+        nF += d.pop;
+        nQ += (int64_t)d.dim * d.dim;
+        nV += d.dim;
+        nM += (int64_t)p.transfer_count * d.dim;
+        nS += (int64_t)((d.pop + 31) & ~31) * RS;
+        const int b = 4 * emtf_lds_floats(d.dim, d.pop, h->nt);
+        if (b > h->group_lds[d.func]) h->group_lds[d.func] = b;
+        if (b > lds) lds = b;
+    }
+    if (lds > kMaxLdsBytes) {
+        free_handle(h);
+        return fail(EMTF_E_UNSUPPORTED, "population tile exceeds LDS (pop x dim too large)");
+    }
+    std::vector<int> groups;
+    for (int f = 0; f < 8; ++f) {
+        h->group_off[f] = (int)groups.size();
+        for (int t = 0; t < n_tasks; ++t)
+            if (tasks[t].func == f) groups.push_back(t);
+    }
+    h->group_off[8] = (int)groups.size();
+    h->mailbox_stride = nM > 0 ? nM : 1;
+
+    std::vector<float> Q((size_t)nQ, 0.0f), V((size_t)nV, 0.0f), TB((size_t)nV, 0.0f);
+    for (int t = 0; t < n_tasks; ++t) {
+        const emtf_task_desc& d = tasks[t];
+        const emtf::TaskDev& td = h->tasks[t];
+        float* q = Q.data() + td.q_off;
+        for (int j = 0; j < d.dim; ++j) {
+            if (d.xform == EMTF_ROTATE) std::memcpy(q + (size_t)j * d.dim, d.rot + (size_t)j * d.dim, sizeof(float) * d.dim);
+            else q[(size_t)j * d.dim + j] = (d.xform == EMTF_DIAG) ? d.diag[j] : 1.0f;
+        }
+        if (d.shift) std::memcpy(V.data() + td.v_off, d.shift, sizeof(float) * d.dim);
+        func_table(d.func, d.dim, TB.data() + td.v_off);
+    }
+
+    auto alloc = [&](float** ptr, int64_t n) -> int {
+        const hipError_t e = hipMalloc((void**)ptr, (size_t)(n > 0 ? n : 1) * sizeof(float));
+        if (e != hipSuccess) return fail(EMTF_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        return EMTF_OK;
+    };
+    int rc = EMTF_OK;
+    if ((rc = alloc(&h->d_X, nX)) || (rc = alloc(&h->d_fit, nF)) || (rc = alloc(&h->d_Q, nQ)) ||
+        (rc = alloc(&h->d_shift, nV)) || (rc = alloc(&h->d_tab, nV)) || (rc = alloc(&h->d_mailbox, 2 * h->mailbox_stride)) ||
+        (rc = alloc(&h->d_scratch, nS))) {
+        free_handle(h);
+        return rc;
+    }
+    hipError_t e = hipMalloc((void**)&h->d_tasks, sizeof(emtf::TaskDev) * n_tasks);
+    if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(tasks)"); }
+    e = hipMalloc((void**)&h->d_groups, sizeof(int) * n_tasks);
+    if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(groups)"); }
+#define CP(dst, src, bytes)                                                                    \
+    do {                                                                                       \
+        hipError_t e2 = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);                     \
+        if (e2 != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, hipGetErrorString(e2)); } \
+    } while (0)
+    CP(h->d_tasks, h->tasks.data(), sizeof(emtf::TaskDev) * n_tasks);
+    CP(h->d_groups, groups.data(), sizeof(int) * n_tasks);
+    CP(h->d_Q, Q.data(), sizeof(float) * Q.size());
+    CP(h->d_shift, V.data(), sizeof(float) * V.size());
+    CP(h->d_tab, TB.data(), sizeof(float) * TB.size());
+#undef CP
+    if ((rc = launch(h, 0, 0, 1, nullptr)) != EMTF_OK) { free_handle(h); return rc; }
+    e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, std::string("init: ") + hipGetErrorString(e)); }
+    *out = h;
+    return EMTF_OK;
+}
+
+int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
+    g_err.clear();
+    if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
+    if (generations < 0) return fail(EMTF_E_INVALID, "generations < 0");
+    hipStream_t s = (hipStream_t)stream;
+    h->stream = s;
+    const int TI = h->p.transfer_interval;
+    const bool xfer = TI > 0 && h->T > 1 && h->p.transfer_count > 0;
+    const int g_end = h->gen + generations;
+    while (h->gen < g_end) {
+        int g1 = g_end;
+        if (xfer) {
+            const int nb = (h->gen / TI + 1) * TI;
+            if (nb < g1) g1 = nb;
+        }
+        const int rc = launch(h, h->gen, g1, 0, s);
+        if (rc != EMTF_OK) return rc;
+        h->gen = g1;
+    }
+    return EMTF_OK;
+}
+
+int emtf_sync(emtf_handle* h) {
+    if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
+    HIP_OR_FAIL(hipStreamSynchronize(h->stream));
+    return EMTF_OK;
+}
+
+int emtf_generation(const emtf_handle* h, int32_t* generation) {
+    if (!h || !generation) return fail(EMTF_E_INVALID, "NULL argument");
+    *generation = h->gen;
+    return EMTF_OK;
+}
+
+int emtf_get_population(emtf_handle* h, int32_t task, float* x, float* fitness) {
+    if (!h || task < 0 || task >= h->T) return fail(EMTF_E_INVALID, "bad handle or task index");
+    HIP_OR_FAIL(hipStreamSynchronize(h->stream));
+    const emtf::TaskDev& td = h->tasks[task];
+    if (x) HIP_OR_FAIL(hipMemcpy(x, h->d_X + td.x_off, sizeof(float) * (size_t)td.pop * td.dim, hipMemcpyDeviceToHost));
+    if (fitness) HIP_OR_FAIL(hipMemcpy(fitness, h->d_fit + td.fit_off, sizeof(float) * (size_t)td.pop, hipMemcpyDeviceToHost));
+    return EMTF_OK;
+}
+
+int emtf_set_population(emtf_handle* h, int32_t task, const float* x, const float* fitness) {
+    if (!h || task < 0 || task >= h->T) return fail(EMTF_E_INVALID, "bad handle or task index");
+    HIP_OR_FAIL(hipStreamSynchronize(h->stream));
+    const emtf::TaskDev& td = h->tasks[task];
+    if (x) HIP_OR_FAIL(hipMemcpy(h->d_X + td.x_off, x, sizeof(float) * (size_t)td.pop * td.dim, hipMemcpyHostToDevice));
+    if (fitness) HIP_OR_FAIL(hipMemcpy(h->d_fit + td.fit_off, fitness, sizeof(float) * (size_t)td.pop, hipMemcpyHostToDevice));
+    return EMTF_OK;
+}
+
+int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_stride) {
+    if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
+    if (x_best && x_stride < h->max_dim) return fail(EMTF_E_INVALID, "x_stride < max dim");
+    HIP_OR_FAIL(hipStreamSynchronize(h->stream));
+    std::vector<float> f, x;
+    for (int t = 0; t < h->T; ++t) {
+        const emtf::TaskDev& td = h->tasks[t];
+        f.resize(td.pop);
+        HIP_OR_FAIL(hipMemcpy(f.data(), h->d_fit + td.fit_off, sizeof(float) * td.pop, hipMemcpyDeviceToHost));
+        int best = 0;
+        for (int i = 1; i < td.pop; ++i) {
+            const float fb = std::isnan(f[best]) ? INFINITY : f[best];
+            const float fi = std::isnan(f[i]) ? INFINITY : f[i];
+            if (fi < fb) best = i;
+        }
+        if (fitness) fitness[t] = f[best];
+        if (x_best)
+            HIP_OR_FAIL(hipMemcpy(x_best + (size_t)t * x_stride, h->d_X + td.x_off + (int64_t)best * td.dim,
+                                  sizeof(float) * td.dim, hipMemcpyDeviceToHost));
+    }
+    return EMTF_OK;
+}
+
+int emtf_destroy(emtf_handle* h) {
+    if (!h) return EMTF_OK;
+    (void)hipStreamSynchronize(h->stream);
+    free_handle(h);
+    return EMTF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,232 @@
+// emtf_math.h -- device arithmetic of the island engine (gfx950).
+//
+// Every function here is specified bit-for-bit: IEEE binary32 with
+// round-to-nearest-even, explicit fmaf only (the library is compiled with
+// -ffp-contract=off), correctly rounded division and sqrt.  The same
+// definitions evaluated on any IEEE host reproduce the device results exactly,
+// which is what the parity tests check.
+//
+// Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11) with
+//   key     = { seed lo32, seed hi32 }
+//   counter = { block, purpose<<24 | sub<<16 | index, generation, task }.
+// It replaces the per-thread cuRAND XORWOW states of the reference
+// (proposed model/SG-EMTF.cu:163-179); draws are addressed by what they are
+// for, never by which thread consumes them.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace emtf {
+
+enum Purpose : uint32_t {
+    P_INIT = 1, P_DE_IDX = 2, P_DE_CR = 3, P_GA_MATE = 4, P_GA_SBX = 5,
+    P_GA_PM = 6, P_MIG = 7, P_XFER = 8, P_ISLX = 9
+};
+
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                     uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+struct Rng {
+    uint32_t k0, k1, task;
+    __device__ __forceinline__ U4 block(uint32_t gen, uint32_t purpose, uint32_t index, uint32_t sub,
+                                        uint32_t blk) const {
+        return philox(blk, (purpose << 24) | ((sub & 0xFFu) << 16) | (index & 0xFFFFu), gen, task, k0, k1);
+    }
+    __device__ __forceinline__ uint32_t word(uint32_t gen, uint32_t purpose, uint32_t index, uint32_t sub,
+                                             uint32_t w) const {
+        const U4 b = block(gen, purpose, index, sub, w >> 2);
+        const uint32_t s = w & 3u;
+        return s == 0 ? b.x : (s == 1 ? b.y : (s == 2 ? b.z : b.w));
+    }
+};
+
+__device__ __forceinline__ uint32_t pick(const U4& b, uint32_t s) {
+    return s == 0 ? b.x : (s == 1 ? b.y : (s == 2 ? b.z : b.w));
+}
+
+// (0,1]: ((w >> 8) + 1) * 2^-24, exact in binary32 (reference: curand_uniform's range)
+__device__ __forceinline__ float uniform01(uint32_t w) {
+    return (float)((w >> 8) + 1u) * 5.9604644775390625e-08f;
+}
+// (0,1) symmetric: (2 (w >> 9) + 1) * 2^-24
+__device__ __forceinline__ float uniform_open(uint32_t w) {
+    return (float)(2u * (w >> 9) + 1u) * 5.9604644775390625e-08f;
+}
+
+__device__ __forceinline__ float pow2i(int n) { return __int_as_float((n + 127) << 23); }
+
+// ---- polynomials (Chebyshev fits rounded to binary32) -----------------------
+// sin^2(pi b) = x S2(x), x = b^2, b in [0, 1/4]; max rel. error 2.9e-8
+__device__ __forceinline__ float poly_s2(float x) {
+    float p = 8.268737793e-01f;
+    p = fmaf(p, x, -3.949177027e+00f);
+    p = fmaf(p, x, 1.321301842e+01f);
+    p = fmaf(p, x, -3.012231827e+01f);
+    p = fmaf(p, x, 4.272840881e+01f);
+    p = fmaf(p, x, -3.246969604e+01f);
+    p = fmaf(p, x, 9.869604111e+00f);
+    return p;
+}
+// sin(2 pi b) = b SN(x), x = b^2, b in [0, 1/4]; max rel. error 5.3e-8
+__device__ __forceinline__ float poly_sn(float x) {
+    float p = 3.665855169e+00f;
+    p = fmaf(p, x, -1.508148289e+01f);
+    p = fmaf(p, x, 4.205813599e+01f);
+    p = fmaf(p, x, -7.670584869e+01f);
+    p = fmaf(p, x, 8.160524750e+01f);
+    p = fmaf(p, x, -4.134170151e+01f);
+    p = fmaf(p, x, 6.283185482e+00f);
+    return p;
+}
+
+// sin^2(pi a) for a in [0, 1/2]
+__device__ __forceinline__ float sin2_half(float a) {
+    const float b = __builtin_fminf(a, 0.5f - a);
+    const float x = b * b;
+    const float y = x * poly_s2(x);
+    return (a > 0.25f) ? (1.0f - y) : y;
+}
+// sin^2(pi v)
+__device__ __forceinline__ float sinpi_sq(float v) {
+    const float r = v - __builtin_rintf(v);
+    return sin2_half(__builtin_fabsf(r));
+}
+// sin(2 pi v)
+__device__ __forceinline__ float sin2pi(float v) {
+    const float r = v - __builtin_rintf(v);
+    const float a = __builtin_fabsf(r);
+    const float b = __builtin_fminf(a, 0.5f - a);
+    const float x = b * b;
+    const float s = b * poly_sn(x);
+    return __builtin_copysignf(s, r);
+}
+
+// e^x - 1: Cody-Waite reduction by ln2 (hi/lo split), polynomial on |r| <= ln2/2
+__device__ __forceinline__ float expm1_f(float x) {
+    if (x != x) return x;
+    if (x < -17.5f) return -1.0f;
+    if (x > 88.0f) return __builtin_inff();
+    const float n = __builtin_rintf(x * 1.44269502e+00f);
+    float r = fmaf(-n, 6.93145752e-01f, x);
+    r = fmaf(-n, 1.42860677e-06f, r);
+    float p = 2.485957884e-05f;
+    p = fmaf(p, r, 1.989927405e-04f);
+    p = fmaf(p, r, 1.388885430e-03f);
+    p = fmaf(p, r, 8.333298378e-03f);
+    p = fmaf(p, r, 4.166666791e-02f);
+    p = fmaf(p, r, 1.666666716e-01f);
+    p = fmaf(p, r, 5.000000000e-01f);
+    const float r2 = r * r;
+    const float em1 = fmaf(r2, p, r);
+    if (n == 0.0f) return em1;
+    const float t = pow2i((int)n);
+    return fmaf(t, em1, t - 1.0f);
+}
+
+__device__ __forceinline__ float exp2_f(float y) {
+    if (y != y) return y;
+    if (y > 128.0f) return __builtin_inff();
+    if (y < -150.0f) return 0.0f;
+    const float n = __builtin_rintf(y);
+    const float f = y - n;
+    float p = 1.530369991e-05f;
+    p = fmaf(p, f, 1.546972926e-04f);
+    p = fmaf(p, f, 1.333347871e-03f);
+    p = fmaf(p, f, 9.618025273e-03f);
+    p = fmaf(p, f, 5.550410971e-02f);
+    p = fmaf(p, f, 2.402265072e-01f);
+    p = fmaf(p, f, 6.931471825e-01f);
+    p = fmaf(p, f, 1.000000000e+00f);
+    const int ni = (int)n;
+    const int n1 = ni / 2;
+    const int n2 = ni - n1;
+    return (p * pow2i(n1)) * pow2i(n2);
+}
+
+__device__ __forceinline__ float log2_f(float x) {
+    if (x != x || x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return -__builtin_inff();
+    if (__builtin_isinf(x)) return x;
+    int adj = 0;
+    if (x < 1.17549435e-38f) { x = x * 16777216.0f; adj = -24; }
+    const uint32_t u = __float_as_uint(x);
+    int e = (int)((u >> 23) & 0xFFu) - 127 + adj;
+    float m = __uint_as_float((u & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421354f) { m = m * 0.5f; e += 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float x2 = s * s;
+    float p = 2.828850150e-01f;
+    p = fmaf(p, x2, 3.199068010e-01f);
+    p = fmaf(p, x2, 4.122092128e-01f);
+    p = fmaf(p, x2, 5.770779252e-01f);
+    p = fmaf(p, x2, 9.617967010e-01f);
+    p = fmaf(p, x2, 2.885390043e+00f);
+    const float t = s * p;
+    return (float)e + t;
+}
+
+// x^y for x >= 0 as 2^(y log2 x)
+__device__ __forceinline__ float pow_f(float x, float y) {
+    if (x == 0.0f) return (y > 0.0f) ? 0.0f : ((y == 0.0f) ? 1.0f : __builtin_inff());
+    const float l = log2_f(x);
+    const float ly = y * l;
+    return exp2_f(ly);
+}
+
+// Weierstrass term sum_{k=0}^{20} 2^(1-k) sin^2(pi 3^k z) with the fractional
+// parts of 3^k |z| formed exactly in 64-bit integers (|z| in [2^-32, 2^23)),
+// truncated to 24 fraction bits.
+__device__ __forceinline__ float weier_term(float z) {
+    if (z != z) return z;
+    const float a = __builtin_fabsf(z);
+    float acc = 0.0f;
+    if (a >= 8388608.0f) return 0.0f;
+    if (a < 2.32830644e-10f) {
+        // 3^k as binary32 (exact for k <= 15, correctly rounded above); 3^k a < 1
+        constexpr float P3F[21] = {1.f, 3.f, 9.f, 27.f, 81.f, 243.f, 729.f, 2187.f, 6561.f, 19683.f,
+                                   59049.f, 177147.f, 531441.f, 1594323.f, 4782969.f, 14348907.f,
+                                   43046721.f, 129140163.f, 387420489.f, 1162261467.f, 3486784401.f};
+#pragma unroll
+        for (int k = 0; k <= 20; ++k) {
+            const float r = a * P3F[k];
+            acc = fmaf(pow2i(1 - k), sin2_half(__builtin_fminf(r, 1.0f - r)), acc);
+        }
+        return acc;
+    }
+    const uint32_t u = __float_as_uint(a);
+    const int ex = (int)((u >> 23) & 0xFFu) - 127;
+    const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
+    const int s = 23 - ex;
+    const uint64_t mask = (((uint64_t)1) << s) - 1u;
+    const int sh = s > 24 ? s - 24 : 0;
+    const float scale = pow2i(-(s - sh));
+    uint64_t y = m & mask;
+#pragma unroll
+    for (int k = 0; k <= 20; ++k) {
+        const uint32_t q = (uint32_t)(y >> sh);
+        const float r = (float)q * scale;
+        acc = fmaf(pow2i(1 - k), sin2_half(__builtin_fminf(r, 1.0f - r)), acc);
+        y = (y * 3u) & mask;
+    }
+    return acc;
+}
+
+}  // namespace emtf

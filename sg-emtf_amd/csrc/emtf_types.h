@@ -1,0 +1,52 @@
+// emtf_types.h -- host/device descriptors of the island engine.
+#pragma once
+#include <stdint.h>
+
+namespace emtf {
+
+enum Func : int32_t {
+    F_SPHERE = 0, F_ROSENBROCK = 1, F_ACKLEY = 2, F_RASTRIGIN = 3,
+    F_GRIEWANK = 4, F_WEIERSTRASS = 5, F_SCHWEFEL = 6, F_ELLIPTIC = 7
+};
+enum Algo : int32_t { A_DE_RAND_1_BIN = 0, A_DE_CURRENT_1_BIN = 1, A_GA_SBX_PM = 2 };
+
+// One task as the kernels see it.  All offsets are in elements.
+struct TaskDev {
+    int32_t func, dim, pop, xform;
+    float lo, hi;
+    int32_t rsv0, rsv1;
+    int64_t x_off;    // population X[pop][dim], row-major
+    int64_t fit_off;  // fitness[pop]
+    int64_t q_off;    // transform matrix Q[dim][dim], row-major (identity / diagonal materialised)
+    int64_t v_off;    // shift[dim] and per-dimension table[dim]
+    int64_t mb_off;   // elite mailbox [transfer_count][dim] inside one mailbox buffer
+    int64_t scr_off;  // scratch [pop][row stride]
+};
+
+struct EngineParams {
+    int32_t algo;
+    float F, CR, eta_c, eta_m, pm;
+    int32_t n_islands, exchange_interval, migrate_interval, migrate_count;
+    int32_t transfer_interval, transfer_count;
+    float transfer_cr;
+    uint32_t seed_lo, seed_hi;
+    int32_t rsv;
+};
+
+struct EpochArgs {
+    const TaskDev* tasks;
+    int32_t n_tasks;
+    int32_t g0, g1;     // generations g0+1 .. g1 are produced
+    int32_t init;       // 1: create and evaluate generation 0 (g0 == g1 == 0)
+    float* X;
+    float* fit;
+    const float* Q;
+    const float* shift;
+    const float* tab;
+    float* mailbox;
+    int64_t mailbox_stride;  // elements per mailbox buffer
+    float* scratch;
+    EngineParams p;
+};
+
+}  // namespace emtf

@@ -1,0 +1,776 @@
+// island.hip -- the island-model evolutionary-multitasking kernel for gfx950.
+//
+// One workgroup (4 wave64s) owns one task for one epoch of generations.  The
+// task's population, its transform matrix and its per-dimension tables live in
+// LDS for the whole epoch; HBM is touched only to load and store the state at
+// the epoch boundaries and for the elite mailbox that carries inter-task
+// transfer.  Per generation and per 32-individual tile a wave
+//   1. draws its Philox numbers in registers (no RNG state in memory),
+//   2. gathers parent/mate rows from LDS and forms the trial vectors,
+//   3. computes z = Q (u - o) with v_mfma_f32_32x32x2_f32: B = the trial tile
+//      (one individual per column, one gene parity per lane half), A = Q,
+//      so each lane ends up holding 16 rows of z for its own individual,
+//   4. evaluates the objective on those rows and combines the two lane halves
+//      with one cross-half shuffle,
+//   5. applies the greedy one-to-one selection in LDS.
+//
+// Reference operators restated (file:line of proposed model/):
+//   GeneticOpera SG-EMTF.cu:1289-1426 / SG-EMTF_DE.cu:1241-1390 (the island loop),
+//   CrossOver/Mutation/Selection SG-EMTF_DE.cu:474-705, SG-EMTF.cu:548-831,
+//   evaluation SG-EMTF_DE.cu:834-1069, IslandMigration SG-EMTF_DE.cu:1093-1170,
+//   SharedToGlobal/SelectFromBlocks SG-EMTF.cu:856-896,1220-1260,
+//   knowledgeTransfer SG-EMTF.cu:1514-1591, pop_init SG-EMTF.cu:146-159.
+//
+// LDS layout of a row (individual i, or row j of Q): the even genes
+// 0,2,4,.. in slots [0, DhP) and the odd genes in [DhP, 2 DhP), so a lane half
+// streams its parity with 16-byte loads; RS = 2 DhP + 4 is the row stride.
+#include <hip/hip_runtime.h>
+#include "emtf_math.h"
+#include "emtf_types.h"
+
+namespace emtf {
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+constexpr int kThreads = 256;
+
+struct Geo {
+    int D, Dh, DhP, RS, pop, popP, ntiles, m;
+};
+
+struct Lds {
+    float* X;    // popP x RS
+    float* Q;    // 32 NT x RS
+    float* o;    // RS
+    float* tab;  // 32 NT
+    float* fit;  // popP
+    float* ftmp; // popP
+    int* ord;    // popP
+    int* aux;    // popP
+};
+
+__host__ __device__ __forceinline__ int lds_floats(int D, int pop, int nt) {
+    const int Dh = (D + 1) >> 1, DhP = (Dh + 3) & ~3, RS = 2 * DhP + 4, popP = (pop + 31) & ~31;
+    return popP * RS + 32 * nt * RS + RS + 32 * nt + 4 * popP;
+}
+
+__device__ __forceinline__ float4 lds4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float f4(const float4& v, int e) { return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w)); }
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) {
+    if (v > hi) v = hi;
+    else if (v < lo) v = lo;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Objective of the lane's individual from its MFMA accumulators.
+// Lane (col, h) holds z_j for j = 32 jt + 8 (r >> 2) + 4 h + (r & 3); the two
+// halves accumulate the even / odd blocks of four rows in increasing j and
+// are combined once.
+// ---------------------------------------------------------------------------
+template <int NT, int FUNC>
+__device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const float* tab, int h) {
+    float s1 = 0.0f, s2 = 0.0f;
+    switch (FUNC) {
+    case F_SPHERE:
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j < D) { const float z = acc[jt][r]; s1 = s1 + z * z; }
+            }
+        break;
+    case F_ELLIPTIC:
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j < D) { const float z = acc[jt][r]; const float zz = z * z; s1 = s1 + tab[j] * zz; }
+            }
+        break;
+    case F_ROSENBROCK: {
+        float nb[NT][4];
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) nb[jt][g] = __shfl_xor(acc[jt][4 * g], 32);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j + 1 < D) {
+                    const int g = r >> 2;
+                    float zn;
+                    if ((r & 3) != 3) zn = acc[jt][r + 1];
+                    else if (h == 0) zn = nb[jt][g];
+                    else zn = (g < 3) ? nb[jt][g + 1] : ((jt + 1 < NT) ? nb[jt + 1 < NT ? jt + 1 : jt][0] : 0.0f);
+                    const float z = acc[jt][r];
+                    const float zz = z * z;
+                    const float a = zz - zn;
+                    const float b = z - 1.0f;
+                    float t1 = 100.0f * a;
+                    t1 = t1 * a;
+                    const float b2 = b * b;
+                    s1 = s1 + (t1 + b2);
+                }
+            }
+        break;
+    }
+    case F_ACKLEY:
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j < D) { const float z = acc[jt][r]; s1 = s1 + z * z; s2 = s2 + sinpi_sq(z); }
+            }
+        break;
+    case F_RASTRIGIN:
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j < D) {
+                    const float z = acc[jt][r];
+                    const float zz = z * z;
+                    const float s20 = 20.0f * sinpi_sq(z);
+                    s1 = s1 + (zz + s20);
+                }
+            }
+        break;
+    case F_GRIEWANK:
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j < D) {
+                    const float z = acc[jt][r];
+                    s1 = s1 + z * z;
+                    const float a = 2.0f * sinpi_sq(z * tab[j]);
+                    const float om = 1.0f - s2;
+                    const float aq = a * om;
+                    s2 = s2 + aq;
+                }
+            }
+        break;
+    case F_WEIERSTRASS:
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j < D) s1 = s1 + weier_term(acc[jt][r]);
+            }
+        break;
+    case F_SCHWEFEL:
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (j < D) {
+                    const float z = acc[jt][r];
+                    const float sq = sqrtf(__builtin_fabsf(z));
+                    const float tv = sq * 0.159154937f;
+                    s1 = s1 + z * sin2pi(tv);
+                }
+            }
+        break;
+    default:
+        return __builtin_nanf("");
+    }
+    const float o1 = __shfl_xor(s1, 32);
+    const float o2 = __shfl_xor(s2, 32);
+    const float S = s1 + o1;  // A + B (commutative)
+    const float Df = (float)D;
+    switch (FUNC) {
+    case F_ACKLEY: {
+        const float C = s2 + o2;
+        const float q = S / Df;
+        const float r = sqrtf(q);
+        const float a = -0.2f * r;
+        const float t1 = -20.0f * expm1_f(a);
+        const float c2 = C / Df;
+        const float b = -2.0f * c2;
+        const float t2 = 2.71828175f * expm1_f(b);
+        return t1 - t2;
+    }
+    case F_GRIEWANK: {
+        const float qA = (h == 0) ? s2 : o2;
+        const float qB = (h == 0) ? o2 : s2;
+        const float om = 1.0f - qA;
+        const float tq = qB * om;
+        const float Qv = qA + tq;
+        const float s4 = S / 4000.0f;
+        return s4 + Qv;
+    }
+    case F_SCHWEFEL: {
+        const float cd = 418.982887f * Df;
+        return cd - S;
+    }
+    default:
+        return S;
+    }
+}
+
+// z = Q w for the 32 individuals of a tile; w holds the lane's parity half.
+template <int NT, int FUNC>
+__device__ __forceinline__ float evaluate_tile(const float (&w)[16 * NT], const Geo& g, const Lds& L, int col, int h) {
+    f16v acc[NT];
+#pragma unroll
+    for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+    const float* qrow = L.Q + col * g.RS + h * g.DhP;
+#pragma unroll
+    for (int sb = 0; sb < 4 * NT; ++sb) {
+        if (4 * sb < g.Dh) {
+            float4 q[NT];
+#pragma unroll
+            for (int jt = 0; jt < NT; ++jt) q[jt] = lds4(qrow + 32 * jt * g.RS + 4 * sb);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (4 * sb + e < g.Dh) {
+#pragma unroll
+                    for (int jt = 0; jt < NT; ++jt)
+                        acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), w[4 * sb + e], acc[jt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    return objective<NT, FUNC>(acc, g.D, L.tab, h);
+}
+
+// ---------------------------------------------------------------------------
+// Trial vectors
+// ---------------------------------------------------------------------------
+struct IdxStream {
+    uint32_t w0, w1, w2, w3;
+    int wi, wn, blk;
+};
+
+__device__ __forceinline__ bool idx_next(IdxStream& st, const Rng& rng, uint32_t gen, uint32_t index, uint32_t& out) {
+    if (st.wi == st.wn) {
+        if (st.blk == 3) return false;
+        st.blk += 1;
+        const U4 b = rng.block(gen, P_DE_IDX, index, 0u, (uint32_t)st.blk);
+        st.w0 = b.x; st.w1 = b.y; st.w2 = b.z; st.w3 = b.w;
+        st.wn = 4;
+        st.wi = 0;
+    }
+    out = st.wi == 0 ? st.w0 : (st.wi == 1 ? st.w1 : (st.wi == 2 ? st.w2 : st.w3));
+    st.wi += 1;
+    return true;
+}
+
+// Rejection draw of an island-local index outside excl[0..nex) (reference
+// CrossOver, SG-EMTF_DE.cu:559-573: redraw curand()%n while it collides).
+__device__ __forceinline__ int draw_excl(IdxStream& st, const Rng& rng, uint32_t gen, uint32_t index, int m,
+                                const int* excl, int nex) {
+    int distinct = 0;
+    for (int a = 0; a < nex; ++a) {
+        int dup = 0;
+        for (int b = 0; b < a; ++b) dup |= (excl[b] == excl[a]);
+        distinct += !dup;
+    }
+    uint32_t w;
+    while (idx_next(st, rng, gen, index, w)) {
+        const int r = (int)(w % (uint32_t)m);
+        int bad = 0;
+        if (distinct < m)
+            for (int a = 0; a < nex; ++a) bad |= (excl[a] == r);
+        if (!bad) return r;
+    }
+    for (int r = 0; r < m; ++r) {
+        int bad = 0;
+        for (int a = 0; a < nex; ++a) bad |= (excl[a] == r);
+        if (!bad) return r;
+    }
+    return 0;
+}
+
+template <int NT>
+__device__ __forceinline__ void trial_de(float (&tr)[16 * NT], const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+                         const Rng& rng, int gen, int ie, int h) {
+    const int m = g.m, il = ie % m, base = ie - il;
+    const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
+    const U4 b0 = rng.block((uint32_t)gen, P_DE_IDX, (uint32_t)ie, 0u, 0u);
+    const int jrand = (int)(b0.w % (uint32_t)g.D);
+    IdxStream st{b0.x, b0.y, b0.z, 0u, 0, 3, 0};
+    int ex[3];
+    int nex = 0;
+    if (rand1) ex[nex++] = il;
+    const int r1 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+    ex[nex++] = r1;
+    const int r2 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+    ex[nex++] = r2;
+    const int r3 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+    const int off = h * g.DhP;
+    const float* xb = L.X + (base + (rand1 ? r1 : il)) * g.RS + off;
+    const float* x2 = L.X + (base + r2) * g.RS + off;
+    const float* x3 = L.X + (base + r3) * g.RS + off;
+    const float* xi = L.X + ie * g.RS + off;
+    const float F = a.p.F, CR = a.p.CR, lo = td.lo, hi = td.hi;
+#pragma unroll
+    for (int sb = 0; sb < 4 * NT; ++sb) {
+        if (4 * sb < g.Dh) {
+            const float4 vb = lds4(xb + 4 * sb), v2 = lds4(x2 + 4 * sb), v3 = lds4(x3 + 4 * sb), vi = lds4(xi + 4 * sb);
+            const U4 cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)sb);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int s = 4 * sb + e, d = 2 * s + h;
+                const float diff = f4(v2, e) - f4(v3, e);
+                const float sc = F * diff;
+                float v = f4(vb, e) + sc;
+                v = clampf(v, lo, hi);
+                const float u = uniform01(pick(cb, (uint32_t)e));
+                float t = (u > CR && d != jrand) ? f4(vi, e) : v;
+                tr[s] = (d < g.D) ? t : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tr[4 * sb + e] = 0.0f;
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void trial_ga(float (&tr)[16 * NT], const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+                         const Rng& rng, int gen, int ie, int h) {
+    const int m = g.m, il = ie % m, base = ie - il;
+    const U4 mb = rng.block((uint32_t)gen, P_GA_MATE, (uint32_t)(ie & ~1), 0u, 0u);
+    const int ra = (int)(pick(mb, (uint32_t)(il & 1)) % (uint32_t)m);
+    const int rb = (int)(pick(mb, (uint32_t)((il & 1) ^ 1)) % (uint32_t)m);
+    const int off = h * g.DhP;
+    const float* xa = L.X + (base + ra) * g.RS + off;
+    const float* xb = L.X + (base + rb) * g.RS + off;
+    const float lo = td.lo, hi = td.hi, rng_ = hi - lo;
+    const float ec = 1.0f / (a.p.eta_c + 1.0f);
+    const float em1 = a.p.eta_m + 1.0f;
+    const float em = 1.0f / em1;
+    const float pm = a.p.pm;
+    const bool odd = (il & 1) != 0;
+#pragma unroll
+    for (int sb = 0; sb < 4 * NT; ++sb) {
+        if (4 * sb < g.Dh) {
+            const float4 va = lds4(xa + 4 * sb), vb = lds4(xb + 4 * sb);
+            const U4 sbk = rng.block((uint32_t)gen, P_GA_SBX, (uint32_t)ie, (uint32_t)h, (uint32_t)sb);
+            const U4 pm0 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb));
+            const U4 pm1 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb + 1));
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int s = 4 * sb + e, d = 2 * s + h;
+                const float p = uniform_open(pick(sbk, (uint32_t)e));
+                const float tv = (p <= 0.5f) ? (2.0f * p) : (0.5f / (1.0f - p));
+                float beta = pow_f(tv, ec);
+                if (odd) beta = -beta;
+                const float t1 = 1.0f + beta;
+                const float t2 = t1 * f4(va, e);
+                const float t3 = 1.0f - beta;
+                const float t4 = t3 * f4(vb, e);
+                const float t5 = t2 + t4;
+                float c = 0.5f * t5;
+                c = clampf(c, lo, hi);
+                const U4& pb = (e < 2) ? pm0 : pm1;
+                const float r = uniform01(pick(pb, (uint32_t)((e & 1) * 2)));
+                if (r < pm) {
+                    const float u = uniform01(pick(pb, (uint32_t)((e & 1) * 2 + 1)));
+                    float delta = 1.0f, iv1, iv2, dl;
+                    const float num = c - lo;
+                    dl = num / rng_;
+                    iv1 = 2.0f * u;
+                    iv2 = 1.0f - iv1;
+                    if (u > 0.5f) {
+                        delta = -1.0f;
+                        const float num2 = hi - c;
+                        dl = num2 / rng_;
+                        const float omu = 1.0f - u;
+                        iv1 = 2.0f * omu;
+                        const float um = u - 0.5f;
+                        iv2 = 2.0f * um;
+                    }
+                    const float omd = 1.0f - dl;
+                    const float pw = pow_f(omd, em1);
+                    const float a2 = iv2 * pw;
+                    const float a3 = iv1 + a2;
+                    const float a4 = pow_f(a3, em);
+                    const float a5 = a4 - 1.0f;
+                    delta = delta * a5;
+                    const float step = delta * rng_;
+                    c = c + step;
+                    c = clampf(c, lo, hi);
+                }
+                tr[s] = (d < g.D) ? c : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tr[4 * sb + e] = 0.0f;
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void shifted(float (&w)[16 * NT], const float (&x)[16 * NT], const Lds& L, const Geo& g, int h) {
+    const float* o = L.o + h * g.DhP;
+#pragma unroll
+    for (int s = 0; s < 16 * NT; ++s) w[s] = (s < g.Dh) ? (x[s] - o[s]) : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Population-level phases (all threads of the workgroup participate)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float key_of(float f) { return (f != f) ? __builtin_inff() : f; }
+
+// ord[q m + rank] = i: the members of each island q ranked by (fitness, index)
+__device__ __forceinline__ void rank_islands(const Lds& L, int pop, int m, int tid) {
+    for (int i = tid; i < pop; i += kThreads) {
+        const int q = i / m, base = q * m, il = i - base;
+        const float fi = key_of(L.fit[i]);
+        int r = 0;
+        for (int p = 0; p < m; ++p) {
+            const float fp = key_of(L.fit[base + p]);
+            r += (fp < fi || (fp == fi && p < il)) ? 1 : 0;
+        }
+        L.ord[base + r] = i;
+    }
+}
+
+template <int NT, int FUNC, int ALGO>
+__device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L, const Rng& rng,
+                                int gen, int tid) {
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
+    const bool multi = g.ntiles > kThreads / 64;
+    float tr[16 * NT];
+    float fchild = 0.0f;
+    int i_keep = -1;
+    bool acc_keep = false;
+    for (int it = wave; it < g.ntiles; it += kThreads / 64) {
+        const int i = it * 32 + col;
+        const bool valid = i < g.pop;
+        const int ie = valid ? i : 0;
+        if constexpr (ALGO == A_GA_SBX_PM) trial_ga<NT>(tr, a, td, g, L, rng, gen, ie, h);
+        else trial_de<NT>(tr, a, td, g, L, rng, gen, ie, h);
+        float w[16 * NT];
+        shifted<NT>(w, tr, L, g, h);
+        const float f = evaluate_tile<NT, FUNC>(w, g, L, col, h);
+        const bool accept = valid && (L.fit[i] >= f);
+        if (multi) {
+            if (accept) {
+                float* dst = a.scratch + td.scr_off + (int64_t)i * g.RS + h * g.DhP;
+#pragma unroll
+                for (int s = 0; s < 16 * NT; ++s)
+                    if (s < g.Dh) dst[s] = tr[s];
+            }
+            if (valid && h == 0) { L.aux[i] = accept ? 1 : 0; L.ftmp[i] = f; }
+        } else {
+            i_keep = i;
+            acc_keep = accept;
+            fchild = f;
+        }
+    }
+    __syncthreads();
+    if (!multi) {
+        if (i_keep >= 0 && acc_keep) {
+            float* dst = L.X + i_keep * g.RS + h * g.DhP;
+#pragma unroll
+            for (int s = 0; s < 16 * NT; ++s)
+                if (s < g.Dh) dst[s] = tr[s];
+            if (h == 0) L.fit[i_keep] = fchild;
+        }
+    } else {
+        const float* src = a.scratch + td.scr_off;
+        for (int e = tid; e < g.pop * g.RS; e += kThreads) {
+            const int row = e / g.RS, c = e - row * g.RS;
+            const int hh = c >= g.DhP ? 1 : 0;
+            const int s = c - hh * g.DhP;
+            if (L.aux[row] && s < g.Dh) L.X[e] = src[e];
+        }
+        for (int i = tid; i < g.pop; i += kThreads)
+            if (L.aux[i]) L.fit[i] = L.ftmp[i];
+    }
+    __syncthreads();
+}
+
+// Cross-island resampling (reference SharedToGlobal + SelectFromBlocks).
+__device__ __forceinline__ void island_exchange(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L, const Rng& rng,
+                                int gen, int tid) {
+    const int n = a.p.n_islands, m = g.m;
+    for (int q = tid; q < n; q += kThreads) {
+        int* arr = L.aux + q * m;
+        for (int p = 0; p < m; ++p) arr[p] = p;
+        for (int k = m - 1; k >= 0; --k) {
+            const uint32_t w = rng.word((uint32_t)gen, P_ISLX, (uint32_t)q, 0u, (uint32_t)(m - 1 - k));
+            const int t = (int)(w % (uint32_t)(k + 1));
+            const int tmp = arr[k];
+            arr[k] = arr[t];
+            arr[t] = tmp;
+        }
+        for (int p = 0; p < m; ++p) L.ord[q * m + arr[p]] = p;  // inverse permutation
+    }
+    __syncthreads();
+    for (int i = tid; i < g.pop; i += kThreads) {
+        const int q = i / m, k = i - q * m;
+        const uint32_t w = rng.word((uint32_t)gen, P_ISLX, (uint32_t)q, 1u, (uint32_t)k);
+        const int src = (q + (int)(w % (uint32_t)n)) % n;
+        L.aux[i] = src * m + L.ord[src * m + k];
+    }
+    __syncthreads();
+    float* scr = a.scratch + td.scr_off;
+    for (int e = tid; e < g.pop * g.RS; e += kThreads) {
+        const int row = e / g.RS, c = e - row * g.RS;
+        scr[e] = L.X[L.aux[row] * g.RS + c];
+    }
+    for (int i = tid; i < g.pop; i += kThreads) L.ftmp[i] = L.fit[L.aux[i]];
+    __syncthreads();
+    for (int e = tid; e < g.pop * g.RS; e += kThreads) L.X[e] = scr[e];
+    for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = L.ftmp[i];
+    __syncthreads();
+}
+
+// Intra-task island migration (reference IslandMigration).
+__device__ __forceinline__ void island_migration(const EpochArgs& a, const Geo& g, const Lds& L, int tid) {
+    const int n = a.p.n_islands, m = g.m, kk = a.p.migrate_count;
+    rank_islands(L, g.pop, m, tid);
+    __syncthreads();
+    for (int e = tid; e < n * kk; e += kThreads) {
+        const int q = e / kk, j = e - q * kk, p = (q + 1) % n;
+        const int src = L.ord[q * m + j], dst = L.ord[p * m + m - kk + j];
+        L.aux[e] = (L.fit[src] < L.fit[dst]) ? 1 : 0;
+    }
+    __syncthreads();
+    for (int e = tid; e < n * kk * g.RS; e += kThreads) {
+        const int pr = e / g.RS, c = e - pr * g.RS;
+        if (L.aux[pr]) {
+            const int q = pr / kk, j = pr - q * kk, p = (q + 1) % n;
+            const int src = L.ord[q * m + j], dst = L.ord[p * m + m - kk + j];
+            L.X[dst * g.RS + c] = L.X[src * g.RS + c];
+        }
+    }
+    for (int e = tid; e < n * kk; e += kThreads) {
+        if (L.aux[e]) {
+            const int q = e / kk, j = e - q * kk, p = (q + 1) % n;
+            const int src = L.ord[q * m + j], dst = L.ord[p * m + m - kk + j];
+            L.fit[dst] = L.fit[src];
+        }
+    }
+    __syncthreads();
+}
+
+// Inter-task transfer at the start of an epoch (reference knowledgeTransfer).
+template <int NT, int FUNC>
+__device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L, const Rng& rng,
+                            int tid) {
+    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0;
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
+    rank_islands(L, g.pop, g.pop, tid);
+    __syncthreads();
+    const uint32_t wd = rng.word((uint32_t)g0, P_MIG, 0u, 0u, 0u);
+    int dn = (int)(wd % (uint32_t)(a.n_tasks - 1));
+    if (dn >= (int)rng.task) dn += 1;
+    const TaskDev dd = a.tasks[dn];
+    const float* mb = a.mailbox + (int64_t)((g0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
+    const bool same_box = (dd.lo == td.lo && dd.hi == td.hi);
+    const float rng_s = td.hi - td.lo, rng_d = dd.hi - dd.lo;
+    const int ntc = (k + 31) / 32;
+    for (int ct = wave; ct < ntc; ct += kThreads / 64) {
+        const int j = ct * 32 + col;
+        const bool valid = j < k;
+        const int je = valid ? j : 0;
+        const float* self = L.X + L.ord[je] * g.RS + h * g.DhP;
+        float cand[16 * NT];
+#pragma unroll
+        for (int s = 0; s < 16 * NT; ++s) {
+            const int d = 2 * s + h;
+            float gv = 0.0f;
+            if (s < g.Dh && d < g.D) {
+                gv = self[s];
+                if (d < dd.dim) {
+                    bool take = true;
+                    if (a.p.transfer_cr > 0.0f) {
+                        const uint32_t wu = rng.word((uint32_t)g0, P_XFER, (uint32_t)je, (uint32_t)h, (uint32_t)s);
+                        take = uniform01(wu) > a.p.transfer_cr;
+                    }
+                    if (take) {
+                        const float v = mb[(int64_t)je * dd.dim + d];
+                        if (same_box) gv = v;
+                        else {
+                            const float t1 = v - dd.lo;
+                            const float t2 = t1 / rng_d;
+                            const float t3 = t2 * rng_s;
+                            gv = td.lo + t3;
+                        }
+                    }
+                }
+            }
+            cand[s] = gv;
+        }
+        float w[16 * NT];
+        shifted<NT>(w, cand, L, g, h);
+        const float f = evaluate_tile<NT, FUNC>(w, g, L, col, h);
+        if (valid) {
+            const int tgt = L.ord[g.pop - k + j];
+            if (f < L.fit[tgt]) {
+                float* dst = L.X + tgt * g.RS + h * g.DhP;
+#pragma unroll
+                for (int s = 0; s < 16 * NT; ++s)
+                    if (s < g.Dh) dst[s] = cand[s];
+                if (h == 0) L.fit[tgt] = f;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int NT, int FUNC, int ALGO>
+__global__ __launch_bounds__(kThreads) void island_epoch_kernel(EpochArgs a, const int* __restrict__ group) {
+    extern __shared__ float smem[];
+    const int task = group[blockIdx.x];
+    const int tid = threadIdx.x;
+    const TaskDev td = a.tasks[task];
+    Geo g;
+    g.D = td.dim;
+    g.Dh = (g.D + 1) >> 1;
+    g.DhP = (g.Dh + 3) & ~3;
+    g.RS = 2 * g.DhP + 4;
+    g.pop = td.pop;
+    g.popP = (g.pop + 31) & ~31;
+    g.ntiles = g.popP / 32;
+    g.m = g.pop / (a.p.n_islands > 0 ? a.p.n_islands : 1);
+    Lds L;
+    L.X = smem;
+    L.Q = L.X + g.popP * g.RS;
+    L.o = L.Q + 32 * NT * g.RS;
+    L.tab = L.o + g.RS;
+    L.fit = L.tab + 32 * NT;
+    L.ftmp = L.fit + g.popP;
+    L.ord = reinterpret_cast<int*>(L.ftmp + g.popP);
+    L.aux = L.ord + g.popP;
+    const Rng rng{a.p.seed_lo, a.p.seed_hi, (uint32_t)task};
+    const int lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
+
+    const int nz = L.fit - smem;  // X, Q, o, tab
+    for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
+    __syncthreads();
+    const float* Qg = a.Q + td.q_off;
+    for (int e = tid; e < g.D * g.D; e += kThreads) {
+        const int j = e / g.D, d = e - j * g.D;
+        L.Q[j * g.RS + (d & 1) * g.DhP + (d >> 1)] = Qg[e];
+    }
+    for (int d = tid; d < g.D; d += kThreads) {
+        L.o[(d & 1) * g.DhP + (d >> 1)] = a.shift[td.v_off + d];
+        L.tab[d] = a.tab[td.v_off + d];
+    }
+    if (a.init) {
+        const float lo = td.lo, rng_ = td.hi - td.lo;
+        for (int e = tid; e < g.pop * g.D; e += kThreads) {
+            const int i = e / g.D, d = e - i * g.D;
+            const float u = uniform01(rng.word(0u, P_INIT, (uint32_t)i, 0u, (uint32_t)d));
+            const float t1 = u * rng_;
+            L.X[i * g.RS + (d & 1) * g.DhP + (d >> 1)] = t1 + lo;
+        }
+        __syncthreads();
+        for (int it = wave; it < g.ntiles; it += kThreads / 64) {
+            const int i = it * 32 + col;
+            float x[16 * NT];
+            const float* row = L.X + i * g.RS + h * g.DhP;
+#pragma unroll
+            for (int s = 0; s < 16 * NT; ++s) x[s] = (s < g.Dh) ? row[s] : 0.0f;
+            float w[16 * NT];
+            shifted<NT>(w, x, L, g, h);
+            const float f = evaluate_tile<NT, FUNC>(w, g, L, col, h);
+            if (i < g.pop && h == 0) L.fit[i] = f;
+        }
+    } else {
+        const float* Xg = a.X + td.x_off;
+        for (int e = tid; e < g.pop * g.D; e += kThreads) {
+            const int i = e / g.D, d = e - i * g.D;
+            L.X[i * g.RS + (d & 1) * g.DhP + (d >> 1)] = Xg[e];
+        }
+        for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = a.fit[td.fit_off + i];
+    }
+    __syncthreads();
+
+    const int TI = a.p.transfer_interval;
+    const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
+    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) transfer_in<NT, FUNC>(a, td, g, L, rng, tid);
+
+    const bool islands = a.p.n_islands > 1;
+    for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
+        if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
+            island_exchange(a, td, g, L, rng, gen, tid);
+        generation_step<NT, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
+        if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
+            island_migration(a, g, L, tid);
+    }
+
+    float* Xg = a.X + td.x_off;
+    for (int e = tid; e < g.pop * g.D; e += kThreads) {
+        const int i = e / g.D, d = e - i * g.D;
+        Xg[e] = L.X[i * g.RS + (d & 1) * g.DhP + (d >> 1)];
+    }
+    for (int i = tid; i < g.pop; i += kThreads) a.fit[td.fit_off + i] = L.fit[i];
+    if (xfer_on && a.g1 > 0 && a.g1 % TI == 0) {
+        __syncthreads();
+        rank_islands(L, g.pop, g.pop, tid);
+        __syncthreads();
+        const int k = a.p.transfer_count;
+        float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
+        for (int e = tid; e < k * g.D; e += kThreads) {
+            const int r = e / g.D, d = e - r * g.D;
+            mb[e] = L.X[L.ord[r] * g.RS + (d & 1) * g.DhP + (d >> 1)];
+        }
+    }
+}
+
+}  // namespace emtf
+
+// ---------------------------------------------------------------------------
+// Launch entry used by the host runtime
+// ---------------------------------------------------------------------------
+#ifndef EMTF_FUNC
+#error "compile island.hip with -DEMTF_FUNC=<objective id>"
+#endif
+
+template <int NT, int ALGO>
+static hipError_t launch_one(const emtf::EpochArgs& a, const int* group, int n, int lds_bytes, hipStream_t stream) {
+    static bool lds_ok = false;
+    auto k = &emtf::island_epoch_kernel<NT, EMTF_FUNC, ALGO>;
+    if (!lds_ok) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        lds_ok = true;
+    }
+    hipLaunchKernelGGL(k, dim3(n), dim3(emtf::kThreads), lds_bytes, stream, a, group);
+    return hipGetLastError();
+}
+
+#define EMTF_CAT2(a, b) a##b
+#define EMTF_CAT(a, b) EMTF_CAT2(a, b)
+
+// One objective per translation unit: emtf_launch_epoch_f<id>.
+extern "C" int EMTF_CAT(emtf_launch_epoch_f, EMTF_FUNC)(const emtf::EpochArgs* args, const int* group, int n,
+                                                         int nt, int lds_bytes, hipStream_t stream) {
+    const bool ga = args->p.algo == emtf::A_GA_SBX_PM;
+    hipError_t e;
+    switch (nt * 2 + (ga ? 1 : 0)) {
+    case 2: e = launch_one<1, emtf::A_DE_RAND_1_BIN>(*args, group, n, lds_bytes, stream); break;
+    case 3: e = launch_one<1, emtf::A_GA_SBX_PM>(*args, group, n, lds_bytes, stream); break;
+    case 4: e = launch_one<2, emtf::A_DE_RAND_1_BIN>(*args, group, n, lds_bytes, stream); break;
+    case 5: e = launch_one<2, emtf::A_GA_SBX_PM>(*args, group, n, lds_bytes, stream); break;
+    case 8: e = launch_one<4, emtf::A_DE_RAND_1_BIN>(*args, group, n, lds_bytes, stream); break;
+    case 9: e = launch_one<4, emtf::A_GA_SBX_PM>(*args, group, n, lds_bytes, stream); break;
+    default: return -1;
+    }
+    return (int)e;
+}
+
+#if EMTF_FUNC == 0
+extern "C" int emtf_lds_floats(int D, int pop, int nt) { return emtf::lds_floats(D, pop, nt); }
+#endif
