@@ -1,0 +1,140 @@
+"""Benchmark: task-generations/s of the island engine on one MI355X.
+
+Workload (the "plain GPU" benchmark configuration): 1024 tasks cycling
+Sphere/Rastrigin/Ackley/Griewank/Rosenbrock/Schwefel/Weierstrass/Elliptic,
+D = 50, z = Q (x - o) with Q from a seeded QR, o ~ U[-80, 80], box
+[-100, 100], pop 128 per task, DE/rand/1/bin F = 0.5 CR = 0.9, 4 elites
+migrating from a uniformly random other task every 20 generations, FP32.
+
+A step is one epoch of 20 generations of every task (one kernel launch);
+the default times 100 steps = the full 2000-generation run with every input
+already resident in HBM.  The CPU baseline is the C oracle (OpenMP across
+tasks) timed on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sg-emtf_amd"))
+
+METRIC = "task-generations/s @1 GPU; time-to-1e-6 error, batched+adjoint vs plain DE"
+PEAK_FP32_MFMA_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="timed epochs of --interval generations")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--tasks", type=int, default=1024)
+    ap.add_argument("--dim", type=int, default=50)
+    ap.add_argument("--pop", type=int, default=128)
+    ap.add_argument("--interval", type=int, default=20, help="generations per epoch = migration interval")
+    ap.add_argument("--elites", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-gens", type=int, default=100, help="generations of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-tasks", type=int, default=1024)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (reference for HBM counters)")
+    return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain torch device copy of known size, profiled beside the kernel to scale FETCH/WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4, "launches": 5}))
+
+
+def cpu_baseline(tasks, params, gens, n_tasks):
+    """Oracle (OpenMP across tasks) on the first n_tasks tasks for `gens` generations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    eng = oracle.Engine(tasks[:n_tasks], **params)
+    t0 = time.perf_counter()
+    eng.run(gens, threads)
+    dt = time.perf_counter() - t0
+    eng.close()
+    return {"value": n_tasks * gens / dt, "unit": "task-generations/s", "cores": threads, "kind": "port",
+            "sample": f"{n_tasks} tasks x {gens} generations of the same workload (oracle, C/OpenMP, "
+                      f"{threads} threads, {dt:.2f} s)"}
+
+
+def main():
+    a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine shards tasks across "
+              f"GPUs only by design, see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+    import emtf
+    from emtf import hip
+    from emtf import tasks as tg
+
+    tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed)
+    params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval,
+                  transfer_count=a.elites, seed=a.seed)
+    eng = emtf.Engine(tasks, **params)
+    for _ in range(a.warmup):
+        eng.run(a.interval)
+    hip.device_synchronize()
+
+    evs = [hip.Event() for _ in range(a.steps + 1)]
+    hip.device_synchronize()
+    t0 = time.perf_counter()
+    evs[0].record()
+    for k in range(a.steps):
+        eng.run(a.interval)  # one kernel launch on the null stream
+        evs[k + 1].record()
+    hip.device_synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [evs[k].elapsed_ms(evs[k + 1]) for k in range(a.steps)]
+    gpu_ms = sum(launch_ms)
+    task_gens = a.tasks * a.interval * a.steps
+    value = task_gens / (gpu_ms / 1e3)
+
+    flop_per_launch = 2.0 * a.pop * a.dim * a.dim * a.tasks * a.interval
+    avg_launch_s = (gpu_ms / a.steps) / 1e3
+    achieved = flop_per_launch / avg_launch_s / 1e12
+    best, _ = eng.best()
+    eng.close()
+
+    cpu = None
+    if a.cpu_gens > 0:
+        cpu = cpu_baseline(tasks, params, a.cpu_gens, min(a.cpu_tasks, a.tasks))
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "task-generations/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": gpu_ms / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain GPU: {a.tasks} rotated tasks, D={a.dim}, pop={a.pop}, DE/rand/1/bin "
+                               f"F=0.5 CR=0.9, {a.elites} elites every {a.interval} gens, "
+                               f"{a.interval * a.steps} timed generations",
+                   "tasks": a.tasks, "dim": a.dim, "pop": a.pop, "generations": a.interval * a.steps,
+                   "generations_per_step": a.interval, "parallelism": "tasks (1 GPU)"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None,
+                     "algorithmic": "2*pop*D^2 FLOP per task-generation (rotation GEMM)",
+                     "avg_launch_ms": gpu_ms / a.steps},
+        "cpu_baseline": cpu,
+        "wall_s": wall,
+        "mean_best_fitness": float(best.mean()),
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,8 +1,8 @@
 """Builds libemtf.so (HIP kernels + C ABI) in-tree for gfx950.
 
-island.hip is compiled once per objective id (-DEMTF_FUNC=0..7) so every
-kernel carries only its own epilogue; the objects are linked with the host
-runtime into sg-emtf_amd/libemtf.so.
+island.hip is compiled once per (tile count, algorithm family) pair; each
+object holds one kernel that dispatches per task to an objective-specialised
+body.  The objects are linked with the host runtime into sg-emtf_amd/libemtf.so.
 """
 import os
 import subprocess
@@ -19,7 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the device compiler away from v_pk_{add,mul,fma}_f32.
 COMMON = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off", "-fPIC",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
-N_FUNCS = 8
+# (tile count NT: dims <= 32 NT; algorithm family: 0 = DE, 2 = GA)
+VARIANTS = [(nt, al) for nt in (1, 2, 4) for al in (0, 2)]
 
 
 def _run(cmd):
@@ -31,8 +32,9 @@ def _run(cmd):
 def build(verbose=False, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 1)
-    units = [([HIPCC] + COMMON + [f"-DEMTF_FUNC={f}", "-c", "-o", os.path.join(OBJ, f"island_f{f}.o"),
-                                  os.path.join(CSRC, "island.hip")]) for f in range(N_FUNCS)]
+    units = [([HIPCC] + COMMON + [f"-DEMTF_NT={nt}", f"-DEMTF_ALGO={al}", "-c",
+                                  "-o", os.path.join(OBJ, f"island_nt{nt}_a{al}.o"), os.path.join(CSRC, "island.hip")])
+             for nt, al in VARIANTS]
     units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, "emtf_host.o"), os.path.join(CSRC, "emtf_host.cpp")])
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(_run, units))
@@ -42,7 +44,7 @@ def build(verbose=False, jobs=None):
             raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
         if verbose and msg:
             print(msg)
-    objs = [os.path.join(OBJ, f"island_f{f}.o") for f in range(N_FUNCS)] + [os.path.join(OBJ, "emtf_host.o")]
+    objs = [os.path.join(OBJ, f"island_nt{nt}_a{al}.o") for nt, al in VARIANTS] + [os.path.join(OBJ, "emtf_host.o")]
     rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
     if rc != 0:
         sys.stderr.write(msg + "\n")

@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,18 +18,34 @@
 #include "../../include/emtf.h"
 #include "emtf_types.h"
 
-// One launcher per objective (island.hip built once per objective id).
-#define EMTF_DECL_LAUNCH(f)                                                                               \
-    extern "C" int emtf_launch_epoch_f##f(const emtf::EpochArgs* args, const int* group, int n, int nt, \
-                                          int lds_bytes, hipStream_t stream);
-EMTF_DECL_LAUNCH(0) EMTF_DECL_LAUNCH(1) EMTF_DECL_LAUNCH(2) EMTF_DECL_LAUNCH(3)
-EMTF_DECL_LAUNCH(4) EMTF_DECL_LAUNCH(5) EMTF_DECL_LAUNCH(6) EMTF_DECL_LAUNCH(7)
+// One launcher per (tile count, algorithm family); island.hip is built once per pair.
+#define EMTF_DECL_LAUNCH(nt, al)                                                                          \
+    extern "C" int emtf_launch_epoch_nt##nt##_a##al(const emtf::EpochArgs* args, const int* order, int n, \
+                                                    int lds_bytes, hipStream_t stream);
+EMTF_DECL_LAUNCH(1, 0) EMTF_DECL_LAUNCH(2, 0) EMTF_DECL_LAUNCH(4, 0)
+EMTF_DECL_LAUNCH(1, 2) EMTF_DECL_LAUNCH(2, 2) EMTF_DECL_LAUNCH(4, 2)
 extern "C" int emtf_lds_floats(int D, int pop, int nt);
 
-typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, int, hipStream_t);
-static const launch_fn kLaunch[8] = {emtf_launch_epoch_f0, emtf_launch_epoch_f1, emtf_launch_epoch_f2,
-                                     emtf_launch_epoch_f3, emtf_launch_epoch_f4, emtf_launch_epoch_f5,
-                                     emtf_launch_epoch_f6, emtf_launch_epoch_f7};
+typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
+
+static launch_fn pick_launcher(int nt, bool ga) {
+    switch (nt) {
+    case 1: return ga ? emtf_launch_epoch_nt1_a2 : emtf_launch_epoch_nt1_a0;
+    case 2: return ga ? emtf_launch_epoch_nt2_a2 : emtf_launch_epoch_nt2_a0;
+    case 4: return ga ? emtf_launch_epoch_nt4_a2 : emtf_launch_epoch_nt4_a0;
+    }
+    return nullptr;
+}
+
+// Relative cost of one generation per (individual, dimension), used to
+// dispatch the expensive tasks first.
+static double func_cost(int f) {
+    switch (f) {
+    case EMTF_WEIERSTRASS: return 21.0;
+    case EMTF_SCHWEFEL: case EMTF_ACKLEY: case EMTF_RASTRIGIN: case EMTF_GRIEWANK: return 2.0;
+    default: return 1.0;
+    }
+}
 
 namespace {
 
@@ -61,9 +78,8 @@ struct emtf_handle {
     float* d_tab = nullptr;
     float* d_mailbox = nullptr;
     float* d_scratch = nullptr;
-    int* d_groups = nullptr;     // task ids grouped by objective
-    int group_off[9] = {0};
-    int group_lds[8] = {0};
+    int* d_order = nullptr;      // task ids, most expensive first
+    int lds_bytes = 0;
     int64_t mailbox_stride = 0;
     int nt = 1;
     int gen = 0;
@@ -81,7 +97,7 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_tab);
     (void)hipFree(h->d_mailbox);
     (void)hipFree(h->d_scratch);
-    (void)hipFree(h->d_groups);
+    (void)hipFree(h->d_order);
     delete h;
 }
 
@@ -111,13 +127,10 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
     a.mailbox_stride = h->mailbox_stride;
     a.scratch = h->d_scratch;
     a.p = h->p;
-    for (int f = 0; f < 8; ++f) {
-        const int n = h->group_off[f + 1] - h->group_off[f];
-        if (n == 0) continue;
-        const int rc = kLaunch[f](&a, h->d_groups + h->group_off[f], n, h->nt, h->group_lds[f], stream);
-        if (rc != 0)
-            return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
-    }
+    launch_fn fn = pick_launcher(h->nt, h->p.algo == emtf::A_GA_SBX_PM);
+    if (!fn) return fail(EMTF_E_UNSUPPORTED, "no kernel built for this tile count");
+    const int rc = fn(&a, h->d_order, h->T, h->lds_bytes, stream);
+    if (rc != 0) return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
     return EMTF_OK;
 }
 
@@ -205,20 +218,21 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         nM += (int64_t)p.transfer_count * d.dim;
         nS += (int64_t)((d.pop + 31) & ~31) * RS;
         const int b = 4 * emtf_lds_floats(d.dim, d.pop, h->nt);
-        if (b > h->group_lds[d.func]) h->group_lds[d.func] = b;
         if (b > lds) lds = b;
     }
     if (lds > kMaxLdsBytes) {
         free_handle(h);
         return fail(EMTF_E_UNSUPPORTED, "population tile exceeds LDS (pop x dim too large)");
     }
-    std::vector<int> groups;
-    for (int f = 0; f < 8; ++f) {
-        h->group_off[f] = (int)groups.size();
-        for (int t = 0; t < n_tasks; ++t)
-            if (tasks[t].func == f) groups.push_back(t);
+    h->lds_bytes = lds;
+    std::vector<int> order(n_tasks);
+    std::vector<double> cost(n_tasks);
+    for (int t = 0; t < n_tasks; ++t) {
+        order[t] = t;
+        cost[t] = func_cost(tasks[t].func) * tasks[t].pop * (double)tasks[t].dim +
+                  2.0 * tasks[t].pop * (double)tasks[t].dim * tasks[t].dim / 64.0;
     }
-    h->group_off[8] = (int)groups.size();
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
     h->mailbox_stride = nM > 0 ? nM : 1;
 
     std::vector<float> Q((size_t)nQ, 0.0f), V((size_t)nV, 0.0f), TB((size_t)nV, 0.0f);
@@ -248,15 +262,15 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     }
     hipError_t e = hipMalloc((void**)&h->d_tasks, sizeof(emtf::TaskDev) * n_tasks);
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(tasks)"); }
-    e = hipMalloc((void**)&h->d_groups, sizeof(int) * n_tasks);
-    if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(groups)"); }
+    e = hipMalloc((void**)&h->d_order, sizeof(int) * n_tasks);
+    if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(order)"); }
 #define CP(dst, src, bytes)                                                                    \
     do {                                                                                       \
         hipError_t e2 = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);                     \
         if (e2 != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, hipGetErrorString(e2)); } \
     } while (0)
     CP(h->d_tasks, h->tasks.data(), sizeof(emtf::TaskDev) * n_tasks);
-    CP(h->d_groups, groups.data(), sizeof(int) * n_tasks);
+    CP(h->d_order, order.data(), sizeof(int) * n_tasks);
     CP(h->d_Q, Q.data(), sizeof(float) * Q.size());
     CP(h->d_shift, V.data(), sizeof(float) * V.size());
     CP(h->d_tab, TB.data(), sizeof(float) * TB.size());

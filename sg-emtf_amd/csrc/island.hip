@@ -628,11 +628,8 @@ __device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& t
 }
 
 template <int NT, int FUNC, int ALGO>
-__global__ __launch_bounds__(kThreads) void island_epoch_kernel(EpochArgs a, const int* __restrict__ group) {
-    extern __shared__ float smem[];
-    const int task = group[blockIdx.x];
+__device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
     const int tid = threadIdx.x;
-    const TaskDev td = a.tasks[task];
     Geo g;
     g.D = td.dim;
     g.Dh = (g.D + 1) >> 1;
@@ -728,49 +725,50 @@ __global__ __launch_bounds__(kThreads) void island_epoch_kernel(EpochArgs a, con
     }
 }
 
+// One workgroup per task; `order` lists the tasks longest-first so the
+// expensive objectives are dispatched before the cheap ones fill the gaps.
+template <int NT, int ALGO>
+__global__ __launch_bounds__(kThreads) void island_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
+    extern __shared__ float smem[];
+    const int task = order[blockIdx.x];
+    const TaskDev td = a.tasks[task];
+    switch (td.func) {
+    case F_SPHERE: epoch_body<NT, F_SPHERE, ALGO>(a, smem, task, td); break;
+    case F_ROSENBROCK: epoch_body<NT, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
+    case F_ACKLEY: epoch_body<NT, F_ACKLEY, ALGO>(a, smem, task, td); break;
+    case F_RASTRIGIN: epoch_body<NT, F_RASTRIGIN, ALGO>(a, smem, task, td); break;
+    case F_GRIEWANK: epoch_body<NT, F_GRIEWANK, ALGO>(a, smem, task, td); break;
+    case F_WEIERSTRASS: epoch_body<NT, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
+    case F_SCHWEFEL: epoch_body<NT, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
+    case F_ELLIPTIC: epoch_body<NT, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
+    default: break;
+    }
+}
+
 }  // namespace emtf
 
-// ---------------------------------------------------------------------------
-// Launch entry used by the host runtime
-// ---------------------------------------------------------------------------
-#ifndef EMTF_FUNC
-#error "compile island.hip with -DEMTF_FUNC=<objective id>"
+#if !defined(EMTF_NT) || !defined(EMTF_ALGO)
+#error "compile island.hip with -DEMTF_NT=<1|2|4> -DEMTF_ALGO=<0|2>"
 #endif
 
-template <int NT, int ALGO>
-static hipError_t launch_one(const emtf::EpochArgs& a, const int* group, int n, int lds_bytes, hipStream_t stream) {
+#define EMTF_CAT3(a, b, c, d) a##b##_##c##d
+#define EMTF_NAME(a, b, c, d) EMTF_CAT3(a, b, c, d)
+
+// emtf_launch_epoch_nt<NT>_a<ALGO>: one instantiation per translation unit.
+extern "C" int EMTF_NAME(emtf_launch_epoch_nt, EMTF_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
+                                                                     int n, int lds_bytes, hipStream_t stream) {
     static bool lds_ok = false;
-    auto k = &emtf::island_epoch_kernel<NT, EMTF_FUNC, ALGO>;
+    auto k = &emtf::island_epoch_kernel<EMTF_NT, EMTF_ALGO>;
     if (!lds_ok) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess) return (int)e;
         lds_ok = true;
     }
-    hipLaunchKernelGGL(k, dim3(n), dim3(emtf::kThreads), lds_bytes, stream, a, group);
-    return hipGetLastError();
+    hipLaunchKernelGGL(k, dim3(n), dim3(emtf::kThreads), lds_bytes, stream, *args, order);
+    return (int)hipGetLastError();
 }
 
-#define EMTF_CAT2(a, b) a##b
-#define EMTF_CAT(a, b) EMTF_CAT2(a, b)
-
-// One objective per translation unit: emtf_launch_epoch_f<id>.
-extern "C" int EMTF_CAT(emtf_launch_epoch_f, EMTF_FUNC)(const emtf::EpochArgs* args, const int* group, int n,
-                                                         int nt, int lds_bytes, hipStream_t stream) {
-    const bool ga = args->p.algo == emtf::A_GA_SBX_PM;
-    hipError_t e;
-    switch (nt * 2 + (ga ? 1 : 0)) {
-    case 2: e = launch_one<1, emtf::A_DE_RAND_1_BIN>(*args, group, n, lds_bytes, stream); break;
-    case 3: e = launch_one<1, emtf::A_GA_SBX_PM>(*args, group, n, lds_bytes, stream); break;
-    case 4: e = launch_one<2, emtf::A_DE_RAND_1_BIN>(*args, group, n, lds_bytes, stream); break;
-    case 5: e = launch_one<2, emtf::A_GA_SBX_PM>(*args, group, n, lds_bytes, stream); break;
-    case 8: e = launch_one<4, emtf::A_DE_RAND_1_BIN>(*args, group, n, lds_bytes, stream); break;
-    case 9: e = launch_one<4, emtf::A_GA_SBX_PM>(*args, group, n, lds_bytes, stream); break;
-    default: return -1;
-    }
-    return (int)e;
-}
-
-#if EMTF_FUNC == 0
+#if EMTF_NT == 1 && EMTF_ALGO == 0
 extern "C" int emtf_lds_floats(int D, int pop, int nt) { return emtf::lds_floats(D, pop, nt); }
 #endif
