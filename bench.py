@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-gens", type=int, default=100, help="generations of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
+    ap.add_argument("--only-func", default=None, help="give every task this objective (per-objective timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters)")
     return ap.parse_args()
@@ -82,7 +83,8 @@ def main():
     from emtf import hip
     from emtf import tasks as tg
 
-    tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed)
+    funcs = [emtf.FUNC_IDS[a.only_func]] * a.tasks if a.only_func else None
+    tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed, funcs=funcs)
     params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval,
                   transfer_count=a.elites, seed=a.seed)
     eng = emtf.Engine(tasks, **params)
@@ -118,7 +120,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "task-generations/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": gpu_ms / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"plain GPU: {a.tasks} rotated tasks, D={a.dim}, pop={a.pop}, DE/rand/1/bin "
+        "config": {"workload": f"plain GPU: {a.tasks} rotated tasks{' (all ' + a.only_func + ')' if a.only_func else ''}, D={a.dim}, pop={a.pop}, DE/rand/1/bin "
                                f"F=0.5 CR=0.9, {a.elites} elites every {a.interval} gens, "
                                f"{a.interval * a.steps} timed generations",
                    "tasks": a.tasks, "dim": a.dim, "pop": a.pop, "generations": a.interval * a.steps,

@@ -29,8 +29,8 @@ def fit(name, f, a, b, n, scale=None):
 
 def main():
     pi = mp.pi
-    # sin^2(pi*b) = x * S2(x), x = b^2, b in [0, 1/4]
-    fit("S2", lambda x: (mp.sin(pi * mp.sqrt(x)) ** 2 / x) if x > 0 else pi ** 2, 0.0, 1.0 / 16, 7)
+    # sin(pi*a) = a * SP(x), x = a^2, a in [0, 1/2]
+    fit("SP", lambda x: (mp.sin(pi * mp.sqrt(x)) / mp.sqrt(x)) if x > 0 else pi, 0.0, 0.25, 5)
     # sin(2*pi*b) = b * SN(x), x = b^2, b in [0, 1/4]
     fit("SN", lambda x: (mp.sin(2 * pi * mp.sqrt(x)) / mp.sqrt(x)) if x > 0 else 2 * pi, 0.0, 1.0 / 16, 7)
     # expm1(r) = r + r^2 * E(r), r in [-ln2/2, ln2/2]

@@ -61,6 +61,11 @@ struct or_engine {
 };
 
 static inline float uniform01(uint32_t w) { return (float)((w >> 8) + 1u) * 5.9604644775390625e-08f; }
+/* 16-bit uniform in (0,1]: half 0 = high 16 bits, half 1 = low 16 bits */
+static inline float uniform16(uint32_t w, int half) {
+    uint32_t v = half ? (w & 0xFFFFu) : (w >> 16);
+    return (float)(v + 1u) * 1.52587890625e-05f;
+}
 static inline float uniform_open(uint32_t w) { return (float)(2u * (w >> 9) + 1u) * 5.9604644775390625e-08f; }
 
 static inline uint32_t draw_word(uint64_t seed, uint32_t task, uint32_t gen, uint32_t purpose,
@@ -202,6 +207,8 @@ static void variation_de(or_engine* e, int ti, int gen) {
         const float* x3 = s->X + (size_t)(base + r3) * D;
         const float* xi = s->X + (size_t)i * D;
         float* tr = s->trial + (size_t)i * D;
+        /* crossover uniforms: parity h, slot s = d >> 1 uses the 16-bit half (s & 1) of
+           word (s >> 1) of sub-stream h (block s >> 3, word (s >> 1) & 3) */
         uint32_t cr[2][4];
         int cached_blk[2] = {-1, -1};
         for (int d = 0; d < D; ++d) {
@@ -209,13 +216,13 @@ static void variation_de(or_engine* e, int ti, int gen) {
             float sc = F * diff;
             float v = xb[d] + sc;
             v = clampf(v, lo, hi);
-            const int h = d & 1, sidx = d >> 1, blk = sidx >> 2;
+            const int h = d & 1, sidx = d >> 1, blk = sidx >> 3;
             if (cached_blk[h] != blk) {
                 or_draw_block(e->p.seed, (uint32_t)ti, (uint32_t)gen, OR_P_DE_CR, (uint32_t)i, (uint32_t)h,
                               (uint32_t)blk, cr[h]);
                 cached_blk[h] = blk;
             }
-            float u = uniform01(cr[h][sidx & 3]);
+            float u = uniform16(cr[h][(sidx >> 1) & 3], sidx & 1);
             tr[d] = (u > CR && d != jrand) ? xi[d] : v;
         }
     }

@@ -51,9 +51,9 @@ static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 /* 2^n for -126 <= n <= 127, exact. */
 static inline float pow2i(int n) { return u2f((uint32_t)(n + 127) << 23); }
 
-/* sin^2(pi b) = x * S2(x), x = b^2, b in [0, 1/4] */
-static const float S2C[7] = {8.268737793e-01f, -3.949177027e+00f, 1.321301842e+01f, -3.012231827e+01f,
-                             4.272840881e+01f, -3.246969604e+01f, 9.869604111e+00f};
+/* sin(pi a) = a * SP(a^2), a in [0, 1/2] */
+static const float SPC[5] = {7.765591145e-02f, -5.982903838e-01f, 2.550077438e+00f, -5.167710304e+00f,
+                             3.141592741e+00f};
 /* sin(2 pi b) = b * SN(x), x = b^2, b in [0, 1/4] */
 static const float SNC[7] = {3.665855169e+00f, -1.508148289e+01f, 4.205813599e+01f, -7.670584869e+01f,
                              8.160524750e+01f, -4.134170151e+01f, 6.283185482e+00f};
@@ -67,19 +67,18 @@ static const float X2C[8] = {1.530369991e-05f, 1.546972926e-04f, 1.333347871e-03
 static const float LC[6] = {2.828850150e-01f, 3.199068010e-01f, 4.122092128e-01f, 5.770779252e-01f,
                             9.617967010e-01f, 2.885390043e+00f};
 
-/* sin^2(pi b), b = |r| in [0, 1/2] */
-static inline float sin2_half(float a) {
-    float b = fminf(a, 0.5f - a);
-    float x = b * b;
-    float p = S2C[0];
-    for (int k = 1; k < 7; ++k) p = fmaf(p, x, S2C[k]);
-    float y = x * p;
-    return (a > 0.25f) ? (1.0f - y) : y;
+/* sin(pi a), a in [0, 1/2] */
+static inline float sinpi_half(float a) {
+    float x = a * a;
+    float p = SPC[0];
+    for (int k = 1; k < 5; ++k) p = fmaf(p, x, SPC[k]);
+    return a * p;
 }
 
 float or_sinpi_sq(float v) {
     float r = v - rintf(v);
-    return sin2_half(fabsf(r));
+    float s = sinpi_half(fabsf(r));
+    return s * s;
 }
 
 float or_sin2pi(float v) {
@@ -157,6 +156,9 @@ static const float P3F[21] = {1.f, 3.f, 9.f, 27.f, 81.f, 243.f, 729.f, 2187.f, 6
                               177147.f, 531441.f, 1594323.f, 4782969.f, 14348907.f, 43046721.f,
                               129140163.f, 387420489.f, 1162261467.f, 3486784401.f};
 
+/* Fractional parts of 3^k |z| for 2^-32 <= |z| < 2^23 are carried as 0.64
+ * fixed-point words y (frac = y / 2^64), so 3^k |z| mod 1 is y_{k+1} = 3 y_k mod 2^64
+ * exactly; each term reads the top 32 bits (correctly rounded to binary32). */
 float or_weier_term(float z) {
     if (z != z) return z;
     float a = fabsf(z);
@@ -165,8 +167,8 @@ float or_weier_term(float z) {
     if (a < 2.32830644e-10f) { /* a < 2^-32: 3^k a < 1, no wrap */
         for (int k = 0; k <= 20; ++k) {
             float r = a * P3F[k];
-            float s = sin2_half(fminf(r, 1.0f - r));
-            acc = fmaf(pow2i(1 - k), s, acc);
+            float s = sinpi_half(fminf(r, 1.0f - r));
+            acc = fmaf(pow2i(1 - k), s * s, acc);
         }
         return acc;
     }
@@ -174,16 +176,13 @@ float or_weier_term(float z) {
     int ex = (int)((u >> 23) & 0xFF) - 127;    /* a in [2^ex, 2^(ex+1)), ex in [-32, 22] */
     uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
     int s = 23 - ex;                            /* fraction bits, 1..55 */
-    uint64_t mask = (((uint64_t)1) << s) - 1;
-    int sh = s > 24 ? s - 24 : 0;
-    float scale = pow2i(-(s - sh));
-    uint64_t y = m & mask;
+    uint64_t y = m << (64 - s);                 /* integer part shifted out */
     for (int k = 0; k <= 20; ++k) {
-        uint32_t q = (uint32_t)(y >> sh);
-        float r = (float)q * scale;
-        float s2 = sin2_half(fminf(r, 1.0f - r));
-        acc = fmaf(pow2i(1 - k), s2, acc);
-        y = (y * 3u) & mask;
+        uint32_t q = (uint32_t)(y >> 32);
+        float r = (float)q * 2.3283064365386963e-10f;
+        float sp = sinpi_half(fminf(r, 1.0f - r));
+        acc = fmaf(pow2i(1 - k), sp * sp, acc);
+        y = y + (y << 1);
     }
     return acc;
 }

@@ -1,6 +1,6 @@
 """Builds libemtf.so (HIP kernels + C ABI) in-tree for gfx950.
 
-island.hip is compiled once per (tile count, algorithm family) pair; each
+island.hip is compiled once per (gene-block count, algorithm family) pair; each
 object holds one kernel that dispatches per task to an objective-specialised
 body.  The objects are linked with the host runtime into sg-emtf_amd/libemtf.so.
 """
@@ -19,8 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the device compiler away from v_pk_{add,mul,fma}_f32.
 COMMON = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off", "-fPIC",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
-# (tile count NT: dims <= 32 NT; algorithm family: 0 = DE, 2 = GA)
-VARIANTS = [(nt, al) for nt in (1, 2, 4) for al in (0, 2)]
+# (gene blocks per parity half KB: dims <= 8 KB; algorithm family: 0 = DE, 2 = GA)
+VARIANTS = [(kb, al) for kb in (2, 4, 7, 8, 12, 13, 16) for al in (0, 2)]
 
 
 def _run(cmd):
@@ -32,9 +32,9 @@ def _run(cmd):
 def build(verbose=False, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 1)
-    units = [([HIPCC] + COMMON + [f"-DEMTF_NT={nt}", f"-DEMTF_ALGO={al}", "-c",
-                                  "-o", os.path.join(OBJ, f"island_nt{nt}_a{al}.o"), os.path.join(CSRC, "island.hip")])
-             for nt, al in VARIANTS]
+    units = [([HIPCC] + COMMON + [f"-DEMTF_KB={kb}", f"-DEMTF_ALGO={al}", "-c",
+                                  "-o", os.path.join(OBJ, f"island_kb{kb}_a{al}.o"), os.path.join(CSRC, "island.hip")])
+             for kb, al in VARIANTS]
     units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, "emtf_host.o"), os.path.join(CSRC, "emtf_host.cpp")])
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(_run, units))
@@ -44,7 +44,7 @@ def build(verbose=False, jobs=None):
             raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
         if verbose and msg:
             print(msg)
-    objs = [os.path.join(OBJ, f"island_nt{nt}_a{al}.o") for nt, al in VARIANTS] + [os.path.join(OBJ, "emtf_host.o")]
+    objs = [os.path.join(OBJ, f"island_kb{kb}_a{al}.o") for kb, al in VARIANTS] + [os.path.join(OBJ, "emtf_host.o")]
     rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
     if rc != 0:
         sys.stderr.write(msg + "\n")

@@ -18,22 +18,34 @@
 #include "../../include/emtf.h"
 #include "emtf_types.h"
 
-// One launcher per (tile count, algorithm family); island.hip is built once per pair.
-#define EMTF_DECL_LAUNCH(nt, al)                                                                          \
-    extern "C" int emtf_launch_epoch_nt##nt##_a##al(const emtf::EpochArgs* args, const int* order, int n, \
+// One launcher per (gene blocks per parity half KB, algorithm family); island.hip is
+// built once per pair.  A batch uses the smallest KB >= ceil(max dim / 8).
+#define EMTF_DECL_LAUNCH(kb, al)                                                                          \
+    extern "C" int emtf_launch_epoch_kb##kb##_a##al(const emtf::EpochArgs* args, const int* order, int n, \
                                                     int lds_bytes, hipStream_t stream);
-EMTF_DECL_LAUNCH(1, 0) EMTF_DECL_LAUNCH(2, 0) EMTF_DECL_LAUNCH(4, 0)
-EMTF_DECL_LAUNCH(1, 2) EMTF_DECL_LAUNCH(2, 2) EMTF_DECL_LAUNCH(4, 2)
-extern "C" int emtf_lds_floats(int D, int pop, int nt);
+#define EMTF_DECL_PAIR(kb) EMTF_DECL_LAUNCH(kb, 0) EMTF_DECL_LAUNCH(kb, 2)
+EMTF_DECL_PAIR(2) EMTF_DECL_PAIR(4) EMTF_DECL_PAIR(7) EMTF_DECL_PAIR(8) EMTF_DECL_PAIR(12) EMTF_DECL_PAIR(13)
+EMTF_DECL_PAIR(16)
+extern "C" int emtf_lds_floats_kb(int pop, int kb);
 
 typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
 
-static launch_fn pick_launcher(int nt, bool ga) {
-    switch (nt) {
-    case 1: return ga ? emtf_launch_epoch_nt1_a2 : emtf_launch_epoch_nt1_a0;
-    case 2: return ga ? emtf_launch_epoch_nt2_a2 : emtf_launch_epoch_nt2_a0;
-    case 4: return ga ? emtf_launch_epoch_nt4_a2 : emtf_launch_epoch_nt4_a0;
+static const int kKB[] = {2, 4, 7, 8, 12, 13, 16};
+
+static int pick_kb(int max_dim) {
+    const int need = (max_dim + 7) / 8;
+    for (int kb : kKB)
+        if (kb >= need) return kb;
+    return -1;
+}
+
+static launch_fn pick_launcher(int kb, bool ga) {
+#define EMTF_CASE(k) \
+    case k: return ga ? emtf_launch_epoch_kb##k##_a2 : emtf_launch_epoch_kb##k##_a0;
+    switch (kb) {
+        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(7) EMTF_CASE(8) EMTF_CASE(12) EMTF_CASE(13) EMTF_CASE(16)
     }
+#undef EMTF_CASE
     return nullptr;
 }
 
@@ -81,7 +93,7 @@ struct emtf_handle {
     int* d_order = nullptr;      // task ids, most expensive first
     int lds_bytes = 0;
     int64_t mailbox_stride = 0;
-    int nt = 1;
+    int kb = 2;
     int gen = 0;
     int max_dim = 0;
     hipStream_t stream = nullptr;
@@ -127,8 +139,8 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
     a.mailbox_stride = h->mailbox_stride;
     a.scratch = h->d_scratch;
     a.p = h->p;
-    launch_fn fn = pick_launcher(h->nt, h->p.algo == emtf::A_GA_SBX_PM);
-    if (!fn) return fail(EMTF_E_UNSUPPORTED, "no kernel built for this tile count");
+    launch_fn fn = pick_launcher(h->kb, h->p.algo == emtf::A_GA_SBX_PM);
+    if (!fn) return fail(EMTF_E_UNSUPPORTED, "no kernel built for this dimension");
     const int rc = fn(&a, h->d_order, h->T, h->lds_bytes, stream);
     if (rc != 0) return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
     return EMTF_OK;
@@ -173,7 +185,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     emtf_handle* h = new emtf_handle();
     h->T = n_tasks;
     h->max_dim = max_dim;
-    h->nt = max_dim <= 32 ? 1 : (max_dim <= 64 ? 2 : 4);
+    h->kb = pick_kb(max_dim);
     emtf::EngineParams& p = h->p;
     p.algo = prm->algo;
     p.F = prm->F;
@@ -210,14 +222,14 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         td.v_off = nV;
         td.mb_off = nM;
         td.scr_off = nS;
-        const int Dh = (d.dim + 1) >> 1, DhP = (Dh + 3) & ~3, RS = 2 * DhP + 4;
+        const int RS = 8 * h->kb + 4;
         nX += (int64_t)d.pop * d.dim;
         nF += d.pop;
         nQ += (int64_t)d.dim * d.dim;
         nV += d.dim;
         nM += (int64_t)p.transfer_count * d.dim;
         nS += (int64_t)((d.pop + 31) & ~31) * RS;
-        const int b = 4 * emtf_lds_floats(d.dim, d.pop, h->nt);
+        const int b = 4 * emtf_lds_floats_kb(d.pop, h->kb);
         if (b > lds) lds = b;
     }
     if (lds > kMaxLdsBytes) {
@@ -338,21 +350,25 @@ int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_strid
     if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
     if (x_best && x_stride < h->max_dim) return fail(EMTF_E_INVALID, "x_stride < max dim");
     HIP_OR_FAIL(hipStreamSynchronize(h->stream));
-    std::vector<float> f, x;
+    const emtf::TaskDev& last = h->tasks.back();
+    std::vector<float> f((size_t)(last.fit_off + last.pop));
+    HIP_OR_FAIL(hipMemcpy(f.data(), h->d_fit, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+    std::vector<float> x;
+    if (x_best) {
+        x.resize((size_t)(last.x_off + (int64_t)last.pop * last.dim));
+        HIP_OR_FAIL(hipMemcpy(x.data(), h->d_X, sizeof(float) * x.size(), hipMemcpyDeviceToHost));
+    }
     for (int t = 0; t < h->T; ++t) {
         const emtf::TaskDev& td = h->tasks[t];
-        f.resize(td.pop);
-        HIP_OR_FAIL(hipMemcpy(f.data(), h->d_fit + td.fit_off, sizeof(float) * td.pop, hipMemcpyDeviceToHost));
+        const float* ft = f.data() + td.fit_off;
         int best = 0;
         for (int i = 1; i < td.pop; ++i) {
-            const float fb = std::isnan(f[best]) ? INFINITY : f[best];
-            const float fi = std::isnan(f[i]) ? INFINITY : f[i];
+            const float fb = std::isnan(ft[best]) ? INFINITY : ft[best];
+            const float fi = std::isnan(ft[i]) ? INFINITY : ft[i];
             if (fi < fb) best = i;
         }
-        if (fitness) fitness[t] = f[best];
-        if (x_best)
-            HIP_OR_FAIL(hipMemcpy(x_best + (size_t)t * x_stride, h->d_X + td.x_off + (int64_t)best * td.dim,
-                                  sizeof(float) * td.dim, hipMemcpyDeviceToHost));
+        if (fitness) fitness[t] = ft[best];
+        if (x_best) std::memcpy(x_best + (size_t)t * x_stride, x.data() + td.x_off + (int64_t)best * td.dim, sizeof(float) * td.dim);
     }
     return EMTF_OK;
 }

@@ -74,16 +74,15 @@ __device__ __forceinline__ float uniform_open(uint32_t w) {
 __device__ __forceinline__ float pow2i(int n) { return __int_as_float((n + 127) << 23); }
 
 // ---- polynomials (Chebyshev fits rounded to binary32) -----------------------
-// sin^2(pi b) = x S2(x), x = b^2, b in [0, 1/4]; max rel. error 2.9e-8
-__device__ __forceinline__ float poly_s2(float x) {
-    float p = 8.268737793e-01f;
-    p = fmaf(p, x, -3.949177027e+00f);
-    p = fmaf(p, x, 1.321301842e+01f);
-    p = fmaf(p, x, -3.012231827e+01f);
-    p = fmaf(p, x, 4.272840881e+01f);
-    p = fmaf(p, x, -3.246969604e+01f);
-    p = fmaf(p, x, 9.869604111e+00f);
-    return p;
+// sin(pi a) = a SP(a^2), a in [0, 1/2]; max rel. error 3.6e-8
+__device__ __forceinline__ float sinpi_half(float a) {
+    const float x = a * a;
+    float p = 7.765591145e-02f;
+    p = fmaf(p, x, -5.982903838e-01f);
+    p = fmaf(p, x, 2.550077438e+00f);
+    p = fmaf(p, x, -5.167710304e+00f);
+    p = fmaf(p, x, 3.141592741e+00f);
+    return a * p;
 }
 // sin(2 pi b) = b SN(x), x = b^2, b in [0, 1/4]; max rel. error 5.3e-8
 __device__ __forceinline__ float poly_sn(float x) {
@@ -97,17 +96,11 @@ __device__ __forceinline__ float poly_sn(float x) {
     return p;
 }
 
-// sin^2(pi a) for a in [0, 1/2]
-__device__ __forceinline__ float sin2_half(float a) {
-    const float b = __builtin_fminf(a, 0.5f - a);
-    const float x = b * b;
-    const float y = x * poly_s2(x);
-    return (a > 0.25f) ? (1.0f - y) : y;
-}
 // sin^2(pi v)
 __device__ __forceinline__ float sinpi_sq(float v) {
     const float r = v - __builtin_rintf(v);
-    return sin2_half(__builtin_fabsf(r));
+    const float sp = sinpi_half(__builtin_fabsf(r));
+    return sp * sp;
 }
 // sin(2 pi v)
 __device__ __forceinline__ float sin2pi(float v) {
@@ -191,9 +184,9 @@ __device__ __forceinline__ float pow_f(float x, float y) {
     return exp2_f(ly);
 }
 
-// Weierstrass term sum_{k=0}^{20} 2^(1-k) sin^2(pi 3^k z) with the fractional
-// parts of 3^k |z| formed exactly in 64-bit integers (|z| in [2^-32, 2^23)),
-// truncated to 24 fraction bits.
+// Weierstrass term sum_{k=0}^{20} 2^(1-k) sin^2(pi 3^k z).  For 2^-32 <= |z| < 2^23
+// the fractional part of 3^k |z| is carried exactly as a 0.64 fixed-point word
+// (3x mod 1 is one 64-bit shift-add); each term reads its top 32 bits.
 __device__ __forceinline__ float weier_term(float z) {
     if (z != z) return z;
     const float a = __builtin_fabsf(z);
@@ -207,7 +200,8 @@ __device__ __forceinline__ float weier_term(float z) {
 #pragma unroll
         for (int k = 0; k <= 20; ++k) {
             const float r = a * P3F[k];
-            acc = fmaf(pow2i(1 - k), sin2_half(__builtin_fminf(r, 1.0f - r)), acc);
+            const float sp = sinpi_half(__builtin_fminf(r, 1.0f - r));
+            acc = fmaf(pow2i(1 - k), sp * sp, acc);
         }
         return acc;
     }
@@ -215,18 +209,22 @@ __device__ __forceinline__ float weier_term(float z) {
     const int ex = (int)((u >> 23) & 0xFFu) - 127;
     const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
     const int s = 23 - ex;
-    const uint64_t mask = (((uint64_t)1) << s) - 1u;
-    const int sh = s > 24 ? s - 24 : 0;
-    const float scale = pow2i(-(s - sh));
-    uint64_t y = m & mask;
+    uint64_t y = m << (64 - s);
 #pragma unroll
     for (int k = 0; k <= 20; ++k) {
-        const uint32_t q = (uint32_t)(y >> sh);
-        const float r = (float)q * scale;
-        acc = fmaf(pow2i(1 - k), sin2_half(__builtin_fminf(r, 1.0f - r)), acc);
-        y = (y * 3u) & mask;
+        const uint32_t q = (uint32_t)(y >> 32);
+        const float r = (float)q * 2.3283064365386963e-10f;
+        const float sp = sinpi_half(__builtin_fminf(r, 1.0f - r));
+        acc = fmaf(pow2i(1 - k), sp * sp, acc);
+        y = y + (y << 1);
     }
     return acc;
+}
+
+// 16-bit uniform in (0,1]: half 0 = high 16 bits of w, half 1 = low 16 bits
+__device__ __forceinline__ float uniform16(uint32_t w, int half) {
+    const uint32_t v = half ? (w & 0xFFFFu) : (w >> 16);
+    return (float)(v + 1u) * 1.52587890625e-05f;
 }
 
 }  // namespace emtf

@@ -22,8 +22,10 @@
 //   knowledgeTransfer SG-EMTF.cu:1514-1591, pop_init SG-EMTF.cu:146-159.
 //
 // LDS layout of a row (individual i, or row j of Q): the even genes
-// 0,2,4,.. in slots [0, DhP) and the odd genes in [DhP, 2 DhP), so a lane half
-// streams its parity with 16-byte loads; RS = 2 DhP + 4 is the row stride.
+// 0,2,4,.. in slots [0, HS) and the odd genes in [HS, 2 HS), HS = 4 KB, so a
+// lane half streams its parity with 16-byte loads; RS = 2 HS + 4 is the row
+// stride.  Slots past the task's dimension hold zeros, which makes every MFMA
+// step and every gene block unconditional: padded steps add +0 products.
 #include <hip/hip_runtime.h>
 #include "emtf_math.h"
 #include "emtf_types.h"
@@ -34,8 +36,16 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
 
+// Row geometry fixed at compile time: KB blocks of 4 genes per parity half.
+template <int KB>
+struct Shape {
+    static constexpr int HS = 4 * KB;          // slots per parity half
+    static constexpr int RS = 2 * HS + 4;      // LDS row stride (floats)
+    static constexpr int NT = (KB + 3) / 4;    // 32-row output tiles (dims <= 32 NT)
+};
+
 struct Geo {
-    int D, Dh, DhP, RS, pop, popP, ntiles, m;
+    int D, Dh, pop, popP, ntiles, m;
 };
 
 struct Lds {
@@ -49,9 +59,9 @@ struct Lds {
     int* aux;    // popP
 };
 
-__host__ __device__ __forceinline__ int lds_floats(int D, int pop, int nt) {
-    const int Dh = (D + 1) >> 1, DhP = (Dh + 3) & ~3, RS = 2 * DhP + 4, popP = (pop + 31) & ~31;
-    return popP * RS + 32 * nt * RS + RS + 32 * nt + 4 * popP;
+__host__ __device__ inline int lds_floats_kb(int pop, int kb) {
+    const int RS = 8 * kb + 4, NT = (kb + 3) / 4, popP = (pop + 31) & ~31;
+    return popP * RS + 32 * NT * RS + RS + 32 * NT + 4 * popP;
 }
 
 __device__ __forceinline__ float4 lds4(const float* p) { return *reinterpret_cast<const float4*>(p); }
@@ -72,26 +82,7 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) {
 template <int NT, int FUNC>
 __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const float* tab, int h) {
     float s1 = 0.0f, s2 = 0.0f;
-    switch (FUNC) {
-    case F_SPHERE:
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j < D) { const float z = acc[jt][r]; s1 = s1 + z * z; }
-            }
-        break;
-    case F_ELLIPTIC:
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j < D) { const float z = acc[jt][r]; const float zz = z * z; s1 = s1 + tab[j] * zz; }
-            }
-        break;
-    case F_ROSENBROCK: {
+    if constexpr (FUNC == F_ROSENBROCK) {
         float nb[NT][4];
 #pragma unroll
         for (int jt = 0; jt < NT; ++jt)
@@ -102,95 +93,68 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j + 1 < D) {
-                    const int g = r >> 2;
-                    float zn;
-                    if ((r & 3) != 3) zn = acc[jt][r + 1];
-                    else if (h == 0) zn = nb[jt][g];
-                    else zn = (g < 3) ? nb[jt][g + 1] : ((jt + 1 < NT) ? nb[jt + 1 < NT ? jt + 1 : jt][0] : 0.0f);
-                    const float z = acc[jt][r];
+                const int g = r >> 2;
+                float zn;
+                if ((r & 3) != 3) zn = acc[jt][r + 1];
+                else if (g < 3) zn = h ? nb[jt][g + 1] : nb[jt][g];
+                else zn = h ? ((jt + 1 < NT) ? nb[(jt + 1) % NT][0] : 0.0f) : nb[jt][g];
+                const float z = acc[jt][r];
+                const float zz = z * z;
+                const float a = zz - zn;
+                const float b = z - 1.0f;
+                float t1 = 100.0f * a;
+                t1 = t1 * a;
+                const float b2 = b * b;
+                const float v = t1 + b2;
+                if (j + 1 < D) s1 = s1 + v;
+            }
+    } else {
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+                const float z = acc[jt][r];
+                if constexpr (FUNC == F_SPHERE) {
+                    const float v = z * z;
+                    if (j < D) s1 = s1 + v;
+                } else if constexpr (FUNC == F_ELLIPTIC) {
                     const float zz = z * z;
-                    const float a = zz - zn;
-                    const float b = z - 1.0f;
-                    float t1 = 100.0f * a;
-                    t1 = t1 * a;
-                    const float b2 = b * b;
-                    s1 = s1 + (t1 + b2);
-                }
-            }
-        break;
-    }
-    case F_ACKLEY:
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j < D) { const float z = acc[jt][r]; s1 = s1 + z * z; s2 = s2 + sinpi_sq(z); }
-            }
-        break;
-    case F_RASTRIGIN:
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j < D) {
-                    const float z = acc[jt][r];
+                    const float v = tab[j] * zz;
+                    if (j < D) s1 = s1 + v;
+                } else if constexpr (FUNC == F_ACKLEY) {
+                    const float v = z * z;
+                    const float sq = sinpi_sq(z);
+                    if (j < D) { s1 = s1 + v; s2 = s2 + sq; }
+                } else if constexpr (FUNC == F_RASTRIGIN) {
                     const float zz = z * z;
                     const float s20 = 20.0f * sinpi_sq(z);
-                    s1 = s1 + (zz + s20);
-                }
-            }
-        break;
-    case F_GRIEWANK:
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j < D) {
-                    const float z = acc[jt][r];
-                    s1 = s1 + z * z;
+                    const float v = zz + s20;
+                    if (j < D) s1 = s1 + v;
+                } else if constexpr (FUNC == F_GRIEWANK) {
+                    const float v = z * z;
                     const float a = 2.0f * sinpi_sq(z * tab[j]);
-                    const float om = 1.0f - s2;
-                    const float aq = a * om;
-                    s2 = s2 + aq;
-                }
-            }
-        break;
-    case F_WEIERSTRASS:
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j < D) s1 = s1 + weier_term(acc[jt][r]);
-            }
-        break;
-    case F_SCHWEFEL:
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (j < D) {
-                    const float z = acc[jt][r];
+                    if (j < D) {
+                        s1 = s1 + v;
+                        const float om = 1.0f - s2;
+                        const float aq = a * om;
+                        s2 = s2 + aq;
+                    }
+                } else if constexpr (FUNC == F_WEIERSTRASS) {
+                    if (j < D) s1 = s1 + weier_term(z);
+                } else if constexpr (FUNC == F_SCHWEFEL) {
                     const float sq = sqrtf(__builtin_fabsf(z));
                     const float tv = sq * 0.159154937f;
-                    s1 = s1 + z * sin2pi(tv);
+                    const float v = z * sin2pi(tv);
+                    if (j < D) s1 = s1 + v;
                 }
             }
-        break;
-    default:
-        return __builtin_nanf("");
     }
     const float o1 = __shfl_xor(s1, 32);
-    const float o2 = __shfl_xor(s2, 32);
     const float S = s1 + o1;  // A + B (commutative)
     const float Df = (float)D;
-    switch (FUNC) {
-    case F_ACKLEY: {
+    if constexpr (FUNC == F_ACKLEY) {
+        const float o2 = __shfl_xor(s2, 32);
         const float C = s2 + o2;
         const float q = S / Df;
         const float r = sqrtf(q);
@@ -200,48 +164,48 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
         const float b = -2.0f * c2;
         const float t2 = 2.71828175f * expm1_f(b);
         return t1 - t2;
-    }
-    case F_GRIEWANK: {
-        const float qA = (h == 0) ? s2 : o2;
-        const float qB = (h == 0) ? o2 : s2;
+    } else if constexpr (FUNC == F_GRIEWANK) {
+        const float o2 = __shfl_xor(s2, 32);
+        const float qA = h ? o2 : s2;
+        const float qB = h ? s2 : o2;
         const float om = 1.0f - qA;
         const float tq = qB * om;
         const float Qv = qA + tq;
         const float s4 = S / 4000.0f;
         return s4 + Qv;
-    }
-    case F_SCHWEFEL: {
+    } else if constexpr (FUNC == F_SCHWEFEL) {
         const float cd = 418.982887f * Df;
         return cd - S;
-    }
-    default:
+    } else {
         return S;
     }
 }
 
-// z = Q w for the 32 individuals of a tile; w holds the lane's parity half.
-template <int NT, int FUNC>
-__device__ __forceinline__ float evaluate_tile(const float (&w)[16 * NT], const Geo& g, const Lds& L, int col, int h) {
+// z = Q (x - o) for the 32 individuals of a tile (x: the lane's parity half)
+// followed by the objective.  4 KB unconditional MFMA steps per output tile.
+template <int KB, int FUNC>
+__device__ __forceinline__ float evaluate_tile(const float (&x)[4 * KB], const Geo& g, const Lds& L, int col, int h) {
+    using S = Shape<KB>;
+    constexpr int NT = S::NT;
     f16v acc[NT];
 #pragma unroll
     for (int jt = 0; jt < NT; ++jt)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
-    const float* qrow = L.Q + col * g.RS + h * g.DhP;
+    const float* qrow = L.Q + col * S::RS + h * S::HS;
+    const float* orow = L.o + h * S::HS;
 #pragma unroll
-    for (int sb = 0; sb < 4 * NT; ++sb) {
-        if (4 * sb < g.Dh) {
-            float4 q[NT];
+    for (int sb = 0; sb < KB; ++sb) {
+        float4 q[NT];
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt) q[jt] = lds4(qrow + 32 * jt * g.RS + 4 * sb);
+        for (int jt = 0; jt < NT; ++jt) q[jt] = lds4(qrow + 32 * jt * S::RS + 4 * sb);
+        const float4 o4 = lds4(orow + 4 * sb);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                if (4 * sb + e < g.Dh) {
+        for (int e = 0; e < 4; ++e) {
+            const float w = x[4 * sb + e] - f4(o4, e);
 #pragma unroll
-                    for (int jt = 0; jt < NT; ++jt)
-                        acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), w[4 * sb + e], acc[jt], 0, 0, 0);
-                }
-            }
+            for (int jt = 0; jt < NT; ++jt)
+                acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), w, acc[jt], 0, 0, 0);
         }
     }
     return objective<NT, FUNC>(acc, g.D, L.tab, h);
@@ -272,7 +236,7 @@ __device__ __forceinline__ bool idx_next(IdxStream& st, const Rng& rng, uint32_t
 // Rejection draw of an island-local index outside excl[0..nex) (reference
 // CrossOver, SG-EMTF_DE.cu:559-573: redraw curand()%n while it collides).
 __device__ __forceinline__ int draw_excl(IdxStream& st, const Rng& rng, uint32_t gen, uint32_t index, int m,
-                                const int* excl, int nex) {
+                                         const int* excl, int nex) {
     int distinct = 0;
     for (int a = 0; a < nex; ++a) {
         int dup = 0;
@@ -295,9 +259,10 @@ __device__ __forceinline__ int draw_excl(IdxStream& st, const Rng& rng, uint32_t
     return 0;
 }
 
-template <int NT>
-__device__ __forceinline__ void trial_de(float (&tr)[16 * NT], const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
-                         const Rng& rng, int gen, int ie, int h) {
+template <int KB>
+__device__ __forceinline__ void trial_de(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
+                                         const Lds& L, const Rng& rng, int gen, int ie, int h) {
+    using S = Shape<KB>;
     const int m = g.m, il = ie % m, base = ie - il;
     const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
     const U4 b0 = rng.block((uint32_t)gen, P_DE_IDX, (uint32_t)ie, 0u, 0u);
@@ -311,45 +276,43 @@ __device__ __forceinline__ void trial_de(float (&tr)[16 * NT], const EpochArgs& 
     const int r2 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
     ex[nex++] = r2;
     const int r3 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
-    const int off = h * g.DhP;
-    const float* xb = L.X + (base + (rand1 ? r1 : il)) * g.RS + off;
-    const float* x2 = L.X + (base + r2) * g.RS + off;
-    const float* x3 = L.X + (base + r3) * g.RS + off;
-    const float* xi = L.X + ie * g.RS + off;
+    const int off = h * S::HS;
+    const float* xb = L.X + (base + (rand1 ? r1 : il)) * S::RS + off;
+    const float* x2 = L.X + (base + r2) * S::RS + off;
+    const float* x3 = L.X + (base + r3) * S::RS + off;
+    const float* xi = L.X + ie * S::RS + off;
     const float F = a.p.F, CR = a.p.CR, lo = td.lo, hi = td.hi;
+    U4 cb{0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int sb = 0; sb < 4 * NT; ++sb) {
-        if (4 * sb < g.Dh) {
-            const float4 vb = lds4(xb + 4 * sb), v2 = lds4(x2 + 4 * sb), v3 = lds4(x3 + 4 * sb), vi = lds4(xi + 4 * sb);
-            const U4 cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)sb);
+    for (int sb = 0; sb < KB; ++sb) {
+        // crossover uniforms: 16-bit halves, two gene blocks per Philox block
+        if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
+        const float4 vb = lds4(xb + 4 * sb), v2 = lds4(x2 + 4 * sb), v3 = lds4(x3 + 4 * sb), vi = lds4(xi + 4 * sb);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int s = 4 * sb + e, d = 2 * s + h;
-                const float diff = f4(v2, e) - f4(v3, e);
-                const float sc = F * diff;
-                float v = f4(vb, e) + sc;
-                v = clampf(v, lo, hi);
-                const float u = uniform01(pick(cb, (uint32_t)e));
-                float t = (u > CR && d != jrand) ? f4(vi, e) : v;
-                tr[s] = (d < g.D) ? t : 0.0f;
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) tr[4 * sb + e] = 0.0f;
+        for (int e = 0; e < 4; ++e) {
+            const int s = 4 * sb + e, d = 2 * s + h;
+            const float diff = f4(v2, e) - f4(v3, e);
+            const float sc = F * diff;
+            float v = f4(vb, e) + sc;
+            v = clampf(v, lo, hi);
+            const float u = uniform16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
+            const float t = (u > CR && d != jrand) ? f4(vi, e) : v;
+            tr[s] = (d < g.D) ? t : 0.0f;
         }
     }
 }
 
-template <int NT>
-__device__ __forceinline__ void trial_ga(float (&tr)[16 * NT], const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
-                         const Rng& rng, int gen, int ie, int h) {
+template <int KB>
+__device__ __forceinline__ void trial_ga(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
+                                         const Lds& L, const Rng& rng, int gen, int ie, int h) {
+    using S = Shape<KB>;
     const int m = g.m, il = ie % m, base = ie - il;
     const U4 mb = rng.block((uint32_t)gen, P_GA_MATE, (uint32_t)(ie & ~1), 0u, 0u);
     const int ra = (int)(pick(mb, (uint32_t)(il & 1)) % (uint32_t)m);
     const int rb = (int)(pick(mb, (uint32_t)((il & 1) ^ 1)) % (uint32_t)m);
-    const int off = h * g.DhP;
-    const float* xa = L.X + (base + ra) * g.RS + off;
-    const float* xb = L.X + (base + rb) * g.RS + off;
+    const int off = h * S::HS;
+    const float* xa = L.X + (base + ra) * S::RS + off;
+    const float* xb = L.X + (base + rb) * S::RS + off;
     const float lo = td.lo, hi = td.hi, rng_ = hi - lo;
     const float ec = 1.0f / (a.p.eta_c + 1.0f);
     const float em1 = a.p.eta_m + 1.0f;
@@ -357,69 +320,73 @@ __device__ __forceinline__ void trial_ga(float (&tr)[16 * NT], const EpochArgs& 
     const float pm = a.p.pm;
     const bool odd = (il & 1) != 0;
 #pragma unroll
-    for (int sb = 0; sb < 4 * NT; ++sb) {
-        if (4 * sb < g.Dh) {
-            const float4 va = lds4(xa + 4 * sb), vb = lds4(xb + 4 * sb);
-            const U4 sbk = rng.block((uint32_t)gen, P_GA_SBX, (uint32_t)ie, (uint32_t)h, (uint32_t)sb);
-            const U4 pm0 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb));
-            const U4 pm1 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb + 1));
+    for (int sb = 0; sb < KB; ++sb) {
+        const float4 va = lds4(xa + 4 * sb), vb = lds4(xb + 4 * sb);
+        const U4 sbk = rng.block((uint32_t)gen, P_GA_SBX, (uint32_t)ie, (uint32_t)h, (uint32_t)sb);
+        const U4 pm0 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb));
+        const U4 pm1 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb + 1));
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int s = 4 * sb + e, d = 2 * s + h;
-                const float p = uniform_open(pick(sbk, (uint32_t)e));
-                const float tv = (p <= 0.5f) ? (2.0f * p) : (0.5f / (1.0f - p));
-                float beta = pow_f(tv, ec);
-                if (odd) beta = -beta;
-                const float t1 = 1.0f + beta;
-                const float t2 = t1 * f4(va, e);
-                const float t3 = 1.0f - beta;
-                const float t4 = t3 * f4(vb, e);
-                const float t5 = t2 + t4;
-                float c = 0.5f * t5;
-                c = clampf(c, lo, hi);
-                const U4& pb = (e < 2) ? pm0 : pm1;
-                const float r = uniform01(pick(pb, (uint32_t)((e & 1) * 2)));
-                if (r < pm) {
-                    const float u = uniform01(pick(pb, (uint32_t)((e & 1) * 2 + 1)));
-                    float delta = 1.0f, iv1, iv2, dl;
-                    const float num = c - lo;
-                    dl = num / rng_;
-                    iv1 = 2.0f * u;
-                    iv2 = 1.0f - iv1;
-                    if (u > 0.5f) {
-                        delta = -1.0f;
-                        const float num2 = hi - c;
-                        dl = num2 / rng_;
-                        const float omu = 1.0f - u;
-                        iv1 = 2.0f * omu;
-                        const float um = u - 0.5f;
-                        iv2 = 2.0f * um;
-                    }
-                    const float omd = 1.0f - dl;
-                    const float pw = pow_f(omd, em1);
-                    const float a2 = iv2 * pw;
-                    const float a3 = iv1 + a2;
-                    const float a4 = pow_f(a3, em);
-                    const float a5 = a4 - 1.0f;
-                    delta = delta * a5;
-                    const float step = delta * rng_;
-                    c = c + step;
-                    c = clampf(c, lo, hi);
+        for (int e = 0; e < 4; ++e) {
+            const int s = 4 * sb + e, d = 2 * s + h;
+            const float p = uniform_open(pick(sbk, (uint32_t)e));
+            const float tv = (p <= 0.5f) ? (2.0f * p) : (0.5f / (1.0f - p));
+            float beta = pow_f(tv, ec);
+            if (odd) beta = -beta;
+            const float t1 = 1.0f + beta;
+            const float t2 = t1 * f4(va, e);
+            const float t3 = 1.0f - beta;
+            const float t4 = t3 * f4(vb, e);
+            const float t5 = t2 + t4;
+            float c = 0.5f * t5;
+            c = clampf(c, lo, hi);
+            const U4& pb = (e < 2) ? pm0 : pm1;
+            const float r = uniform01(pick(pb, (uint32_t)((e & 1) * 2)));
+            if (r < pm) {
+                const float u = uniform01(pick(pb, (uint32_t)((e & 1) * 2 + 1)));
+                float delta = 1.0f, iv1, iv2, dl;
+                const float num = c - lo;
+                dl = num / rng_;
+                iv1 = 2.0f * u;
+                iv2 = 1.0f - iv1;
+                if (u > 0.5f) {
+                    delta = -1.0f;
+                    const float num2 = hi - c;
+                    dl = num2 / rng_;
+                    const float omu = 1.0f - u;
+                    iv1 = 2.0f * omu;
+                    const float um = u - 0.5f;
+                    iv2 = 2.0f * um;
                 }
-                tr[s] = (d < g.D) ? c : 0.0f;
+                const float omd = 1.0f - dl;
+                const float pw = pow_f(omd, em1);
+                const float a2 = iv2 * pw;
+                const float a3 = iv1 + a2;
+                const float a4 = pow_f(a3, em);
+                const float a5 = a4 - 1.0f;
+                delta = delta * a5;
+                const float step = delta * rng_;
+                c = c + step;
+                c = clampf(c, lo, hi);
             }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) tr[4 * sb + e] = 0.0f;
+            tr[s] = (d < g.D) ? c : 0.0f;
         }
     }
 }
 
-template <int NT>
-__device__ __forceinline__ void shifted(float (&w)[16 * NT], const float (&x)[16 * NT], const Lds& L, const Geo& g, int h) {
-    const float* o = L.o + h * g.DhP;
+template <int KB>
+__device__ __forceinline__ void load_half_row(float (&x)[4 * KB], const float* row) {
 #pragma unroll
-    for (int s = 0; s < 16 * NT; ++s) w[s] = (s < g.Dh) ? (x[s] - o[s]) : 0.0f;
+    for (int sb = 0; sb < KB; ++sb) {
+        const float4 v = lds4(row + 4 * sb);
+        x[4 * sb] = v.x; x[4 * sb + 1] = v.y; x[4 * sb + 2] = v.z; x[4 * sb + 3] = v.w;
+    }
+}
+
+template <int KB>
+__device__ __forceinline__ void store_half_row(float* row, const float (&x)[4 * KB]) {
+#pragma unroll
+    for (int sb = 0; sb < KB; ++sb)
+        *reinterpret_cast<float4*>(row + 4 * sb) = make_float4(x[4 * sb], x[4 * sb + 1], x[4 * sb + 2], x[4 * sb + 3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,12 +408,13 @@ __device__ __forceinline__ void rank_islands(const Lds& L, int pop, int m, int t
     }
 }
 
-template <int NT, int FUNC, int ALGO>
-__device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L, const Rng& rng,
-                                int gen, int tid) {
+template <int KB, int FUNC, int ALGO>
+__device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+                                                const Rng& rng, int gen, int tid) {
+    using S = Shape<KB>;
     const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
     const bool multi = g.ntiles > kThreads / 64;
-    float tr[16 * NT];
+    float tr[4 * KB];
     float fchild = 0.0f;
     int i_keep = -1;
     bool acc_keep = false;
@@ -454,19 +422,12 @@ __device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDe
         const int i = it * 32 + col;
         const bool valid = i < g.pop;
         const int ie = valid ? i : 0;
-        if constexpr (ALGO == A_GA_SBX_PM) trial_ga<NT>(tr, a, td, g, L, rng, gen, ie, h);
-        else trial_de<NT>(tr, a, td, g, L, rng, gen, ie, h);
-        float w[16 * NT];
-        shifted<NT>(w, tr, L, g, h);
-        const float f = evaluate_tile<NT, FUNC>(w, g, L, col, h);
+        if constexpr (ALGO == A_GA_SBX_PM) trial_ga<KB>(tr, a, td, g, L, rng, gen, ie, h);
+        else trial_de<KB>(tr, a, td, g, L, rng, gen, ie, h);
+        const float f = evaluate_tile<KB, FUNC>(tr, g, L, col, h);
         const bool accept = valid && (L.fit[i] >= f);
         if (multi) {
-            if (accept) {
-                float* dst = a.scratch + td.scr_off + (int64_t)i * g.RS + h * g.DhP;
-#pragma unroll
-                for (int s = 0; s < 16 * NT; ++s)
-                    if (s < g.Dh) dst[s] = tr[s];
-            }
+            if (accept) store_half_row<KB>(a.scratch + td.scr_off + (int64_t)i * S::RS + h * S::HS, tr);
             if (valid && h == 0) { L.aux[i] = accept ? 1 : 0; L.ftmp[i] = f; }
         } else {
             i_keep = i;
@@ -477,19 +438,18 @@ __device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDe
     __syncthreads();
     if (!multi) {
         if (i_keep >= 0 && acc_keep) {
-            float* dst = L.X + i_keep * g.RS + h * g.DhP;
-#pragma unroll
-            for (int s = 0; s < 16 * NT; ++s)
-                if (s < g.Dh) dst[s] = tr[s];
+            store_half_row<KB>(L.X + i_keep * S::RS + h * S::HS, tr);
             if (h == 0) L.fit[i_keep] = fchild;
         }
     } else {
         const float* src = a.scratch + td.scr_off;
-        for (int e = tid; e < g.pop * g.RS; e += kThreads) {
-            const int row = e / g.RS, c = e - row * g.RS;
-            const int hh = c >= g.DhP ? 1 : 0;
-            const int s = c - hh * g.DhP;
-            if (L.aux[row] && s < g.Dh) L.X[e] = src[e];
+        for (int e = tid; e < g.pop * (2 * KB); e += kThreads) {
+            const int row = e / (2 * KB), blk = e - row * (2 * KB);
+            if (L.aux[row]) {
+                const int c = (blk < KB) ? 4 * blk : S::HS + 4 * (blk - KB);
+                *reinterpret_cast<float4*>(L.X + row * S::RS + c) =
+                    *reinterpret_cast<const float4*>(src + (int64_t)row * S::RS + c);
+            }
         }
         for (int i = tid; i < g.pop; i += kThreads)
             if (L.aux[i]) L.fit[i] = L.ftmp[i];
@@ -498,8 +458,10 @@ __device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDe
 }
 
 // Cross-island resampling (reference SharedToGlobal + SelectFromBlocks).
-__device__ __forceinline__ void island_exchange(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L, const Rng& rng,
-                                int gen, int tid) {
+template <int KB>
+__device__ __forceinline__ void island_exchange(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+                                                const Rng& rng, int gen, int tid) {
+    constexpr int RS = Shape<KB>::RS;
     const int n = a.p.n_islands, m = g.m;
     for (int q = tid; q < n; q += kThreads) {
         int* arr = L.aux + q * m;
@@ -522,19 +484,21 @@ __device__ __forceinline__ void island_exchange(const EpochArgs& a, const TaskDe
     }
     __syncthreads();
     float* scr = a.scratch + td.scr_off;
-    for (int e = tid; e < g.pop * g.RS; e += kThreads) {
-        const int row = e / g.RS, c = e - row * g.RS;
-        scr[e] = L.X[L.aux[row] * g.RS + c];
+    for (int e = tid; e < g.pop * RS; e += kThreads) {
+        const int row = e / RS, c = e - row * RS;
+        scr[e] = L.X[L.aux[row] * RS + c];
     }
     for (int i = tid; i < g.pop; i += kThreads) L.ftmp[i] = L.fit[L.aux[i]];
     __syncthreads();
-    for (int e = tid; e < g.pop * g.RS; e += kThreads) L.X[e] = scr[e];
+    for (int e = tid; e < g.pop * RS; e += kThreads) L.X[e] = scr[e];
     for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = L.ftmp[i];
     __syncthreads();
 }
 
 // Intra-task island migration (reference IslandMigration).
+template <int KB>
 __device__ __forceinline__ void island_migration(const EpochArgs& a, const Geo& g, const Lds& L, int tid) {
+    constexpr int RS = Shape<KB>::RS;
     const int n = a.p.n_islands, m = g.m, kk = a.p.migrate_count;
     rank_islands(L, g.pop, m, tid);
     __syncthreads();
@@ -544,12 +508,12 @@ __device__ __forceinline__ void island_migration(const EpochArgs& a, const Geo& 
         L.aux[e] = (L.fit[src] < L.fit[dst]) ? 1 : 0;
     }
     __syncthreads();
-    for (int e = tid; e < n * kk * g.RS; e += kThreads) {
-        const int pr = e / g.RS, c = e - pr * g.RS;
+    for (int e = tid; e < n * kk * RS; e += kThreads) {
+        const int pr = e / RS, c = e - pr * RS;
         if (L.aux[pr]) {
             const int q = pr / kk, j = pr - q * kk, p = (q + 1) % n;
             const int src = L.ord[q * m + j], dst = L.ord[p * m + m - kk + j];
-            L.X[dst * g.RS + c] = L.X[src * g.RS + c];
+            L.X[dst * RS + c] = L.X[src * RS + c];
         }
     }
     for (int e = tid; e < n * kk; e += kThreads) {
@@ -563,9 +527,10 @@ __device__ __forceinline__ void island_migration(const EpochArgs& a, const Geo& 
 }
 
 // Inter-task transfer at the start of an epoch (reference knowledgeTransfer).
-template <int NT, int FUNC>
-__device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L, const Rng& rng,
-                            int tid) {
+template <int KB, int FUNC>
+__device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+                                            const Rng& rng, int tid) {
+    using S = Shape<KB>;
     const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0;
     const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
     rank_islands(L, g.pop, g.pop, tid);
@@ -582,44 +547,34 @@ __device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& t
         const int j = ct * 32 + col;
         const bool valid = j < k;
         const int je = valid ? j : 0;
-        const float* self = L.X + L.ord[je] * g.RS + h * g.DhP;
-        float cand[16 * NT];
+        float cand[S::HS];
+        load_half_row<KB>(cand, L.X + L.ord[je] * S::RS + h * S::HS);
 #pragma unroll
-        for (int s = 0; s < 16 * NT; ++s) {
+        for (int s = 0; s < S::HS; ++s) {
             const int d = 2 * s + h;
-            float gv = 0.0f;
-            if (s < g.Dh && d < g.D) {
-                gv = self[s];
-                if (d < dd.dim) {
-                    bool take = true;
-                    if (a.p.transfer_cr > 0.0f) {
-                        const uint32_t wu = rng.word((uint32_t)g0, P_XFER, (uint32_t)je, (uint32_t)h, (uint32_t)s);
-                        take = uniform01(wu) > a.p.transfer_cr;
-                    }
-                    if (take) {
-                        const float v = mb[(int64_t)je * dd.dim + d];
-                        if (same_box) gv = v;
-                        else {
-                            const float t1 = v - dd.lo;
-                            const float t2 = t1 / rng_d;
-                            const float t3 = t2 * rng_s;
-                            gv = td.lo + t3;
-                        }
+            if (d < g.D && d < dd.dim) {
+                bool take = true;
+                if (a.p.transfer_cr > 0.0f) {
+                    const uint32_t wu = rng.word((uint32_t)g0, P_XFER, (uint32_t)je, (uint32_t)h, (uint32_t)s);
+                    take = uniform01(wu) > a.p.transfer_cr;
+                }
+                if (take) {
+                    const float v = mb[(int64_t)je * dd.dim + d];
+                    if (same_box) cand[s] = v;
+                    else {
+                        const float t1 = v - dd.lo;
+                        const float t2 = t1 / rng_d;
+                        const float t3 = t2 * rng_s;
+                        cand[s] = td.lo + t3;
                     }
                 }
             }
-            cand[s] = gv;
         }
-        float w[16 * NT];
-        shifted<NT>(w, cand, L, g, h);
-        const float f = evaluate_tile<NT, FUNC>(w, g, L, col, h);
+        const float f = evaluate_tile<KB, FUNC>(cand, g, L, col, h);
         if (valid) {
             const int tgt = L.ord[g.pop - k + j];
             if (f < L.fit[tgt]) {
-                float* dst = L.X + tgt * g.RS + h * g.DhP;
-#pragma unroll
-                for (int s = 0; s < 16 * NT; ++s)
-                    if (s < g.Dh) dst[s] = cand[s];
+                store_half_row<KB>(L.X + tgt * S::RS + h * S::HS, cand);
                 if (h == 0) L.fit[tgt] = f;
             }
         }
@@ -627,23 +582,23 @@ __device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& t
     __syncthreads();
 }
 
-template <int NT, int FUNC, int ALGO>
+template <int KB, int FUNC, int ALGO>
 __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
+    using S = Shape<KB>;
+    constexpr int RS = S::RS, HS = S::HS, NT = S::NT;
     const int tid = threadIdx.x;
     Geo g;
     g.D = td.dim;
     g.Dh = (g.D + 1) >> 1;
-    g.DhP = (g.Dh + 3) & ~3;
-    g.RS = 2 * g.DhP + 4;
     g.pop = td.pop;
     g.popP = (g.pop + 31) & ~31;
     g.ntiles = g.popP / 32;
     g.m = g.pop / (a.p.n_islands > 0 ? a.p.n_islands : 1);
     Lds L;
     L.X = smem;
-    L.Q = L.X + g.popP * g.RS;
-    L.o = L.Q + 32 * NT * g.RS;
-    L.tab = L.o + g.RS;
+    L.Q = L.X + g.popP * RS;
+    L.o = L.Q + 32 * NT * RS;
+    L.tab = L.o + RS;
     L.fit = L.tab + 32 * NT;
     L.ftmp = L.fit + g.popP;
     L.ord = reinterpret_cast<int*>(L.ftmp + g.popP);
@@ -657,10 +612,10 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
     const float* Qg = a.Q + td.q_off;
     for (int e = tid; e < g.D * g.D; e += kThreads) {
         const int j = e / g.D, d = e - j * g.D;
-        L.Q[j * g.RS + (d & 1) * g.DhP + (d >> 1)] = Qg[e];
+        L.Q[j * RS + (d & 1) * HS + (d >> 1)] = Qg[e];
     }
     for (int d = tid; d < g.D; d += kThreads) {
-        L.o[(d & 1) * g.DhP + (d >> 1)] = a.shift[td.v_off + d];
+        L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
         L.tab[d] = a.tab[td.v_off + d];
     }
     if (a.init) {
@@ -669,25 +624,21 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
             const int i = e / g.D, d = e - i * g.D;
             const float u = uniform01(rng.word(0u, P_INIT, (uint32_t)i, 0u, (uint32_t)d));
             const float t1 = u * rng_;
-            L.X[i * g.RS + (d & 1) * g.DhP + (d >> 1)] = t1 + lo;
+            L.X[i * RS + (d & 1) * HS + (d >> 1)] = t1 + lo;
         }
         __syncthreads();
         for (int it = wave; it < g.ntiles; it += kThreads / 64) {
             const int i = it * 32 + col;
-            float x[16 * NT];
-            const float* row = L.X + i * g.RS + h * g.DhP;
-#pragma unroll
-            for (int s = 0; s < 16 * NT; ++s) x[s] = (s < g.Dh) ? row[s] : 0.0f;
-            float w[16 * NT];
-            shifted<NT>(w, x, L, g, h);
-            const float f = evaluate_tile<NT, FUNC>(w, g, L, col, h);
+            float x[HS];
+            load_half_row<KB>(x, L.X + i * RS + h * HS);
+            const float f = evaluate_tile<KB, FUNC>(x, g, L, col, h);
             if (i < g.pop && h == 0) L.fit[i] = f;
         }
     } else {
         const float* Xg = a.X + td.x_off;
         for (int e = tid; e < g.pop * g.D; e += kThreads) {
             const int i = e / g.D, d = e - i * g.D;
-            L.X[i * g.RS + (d & 1) * g.DhP + (d >> 1)] = Xg[e];
+            L.X[i * RS + (d & 1) * HS + (d >> 1)] = Xg[e];
         }
         for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = a.fit[td.fit_off + i];
     }
@@ -695,21 +646,21 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
 
     const int TI = a.p.transfer_interval;
     const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
-    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) transfer_in<NT, FUNC>(a, td, g, L, rng, tid);
+    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) transfer_in<KB, FUNC>(a, td, g, L, rng, tid);
 
     const bool islands = a.p.n_islands > 1;
     for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
         if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
-            island_exchange(a, td, g, L, rng, gen, tid);
-        generation_step<NT, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
+            island_exchange<KB>(a, td, g, L, rng, gen, tid);
+        generation_step<KB, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
         if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
-            island_migration(a, g, L, tid);
+            island_migration<KB>(a, g, L, tid);
     }
 
     float* Xg = a.X + td.x_off;
     for (int e = tid; e < g.pop * g.D; e += kThreads) {
         const int i = e / g.D, d = e - i * g.D;
-        Xg[e] = L.X[i * g.RS + (d & 1) * g.DhP + (d >> 1)];
+        Xg[e] = L.X[i * RS + (d & 1) * HS + (d >> 1)];
     }
     for (int i = tid; i < g.pop; i += kThreads) a.fit[td.fit_off + i] = L.fit[i];
     if (xfer_on && a.g1 > 0 && a.g1 % TI == 0) {
@@ -720,45 +671,45 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
         float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
         for (int e = tid; e < k * g.D; e += kThreads) {
             const int r = e / g.D, d = e - r * g.D;
-            mb[e] = L.X[L.ord[r] * g.RS + (d & 1) * g.DhP + (d >> 1)];
+            mb[e] = L.X[L.ord[r] * RS + (d & 1) * HS + (d >> 1)];
         }
     }
 }
 
 // One workgroup per task; `order` lists the tasks longest-first so the
 // expensive objectives are dispatched before the cheap ones fill the gaps.
-template <int NT, int ALGO>
+template <int KB, int ALGO>
 __global__ __launch_bounds__(kThreads) void island_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
     extern __shared__ float smem[];
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
     switch (td.func) {
-    case F_SPHERE: epoch_body<NT, F_SPHERE, ALGO>(a, smem, task, td); break;
-    case F_ROSENBROCK: epoch_body<NT, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
-    case F_ACKLEY: epoch_body<NT, F_ACKLEY, ALGO>(a, smem, task, td); break;
-    case F_RASTRIGIN: epoch_body<NT, F_RASTRIGIN, ALGO>(a, smem, task, td); break;
-    case F_GRIEWANK: epoch_body<NT, F_GRIEWANK, ALGO>(a, smem, task, td); break;
-    case F_WEIERSTRASS: epoch_body<NT, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
-    case F_SCHWEFEL: epoch_body<NT, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
-    case F_ELLIPTIC: epoch_body<NT, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
+    case F_SPHERE: epoch_body<KB, F_SPHERE, ALGO>(a, smem, task, td); break;
+    case F_ROSENBROCK: epoch_body<KB, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
+    case F_ACKLEY: epoch_body<KB, F_ACKLEY, ALGO>(a, smem, task, td); break;
+    case F_RASTRIGIN: epoch_body<KB, F_RASTRIGIN, ALGO>(a, smem, task, td); break;
+    case F_GRIEWANK: epoch_body<KB, F_GRIEWANK, ALGO>(a, smem, task, td); break;
+    case F_WEIERSTRASS: epoch_body<KB, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
+    case F_SCHWEFEL: epoch_body<KB, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
+    case F_ELLIPTIC: epoch_body<KB, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
     default: break;
     }
 }
 
 }  // namespace emtf
 
-#if !defined(EMTF_NT) || !defined(EMTF_ALGO)
-#error "compile island.hip with -DEMTF_NT=<1|2|4> -DEMTF_ALGO=<0|2>"
+#if !defined(EMTF_KB) || !defined(EMTF_ALGO)
+#error "compile island.hip with -DEMTF_KB=<gene blocks per half> -DEMTF_ALGO=<0|2>"
 #endif
 
 #define EMTF_CAT3(a, b, c, d) a##b##_##c##d
 #define EMTF_NAME(a, b, c, d) EMTF_CAT3(a, b, c, d)
 
-// emtf_launch_epoch_nt<NT>_a<ALGO>: one instantiation per translation unit.
-extern "C" int EMTF_NAME(emtf_launch_epoch_nt, EMTF_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
+// emtf_launch_epoch_kb<KB>_a<ALGO>: one instantiation per translation unit.
+extern "C" int EMTF_NAME(emtf_launch_epoch_kb, EMTF_KB, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
                                                                      int n, int lds_bytes, hipStream_t stream) {
     static bool lds_ok = false;
-    auto k = &emtf::island_epoch_kernel<EMTF_NT, EMTF_ALGO>;
+    auto k = &emtf::island_epoch_kernel<EMTF_KB, EMTF_ALGO>;
     if (!lds_ok) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -769,6 +720,6 @@ extern "C" int EMTF_NAME(emtf_launch_epoch_nt, EMTF_NT, a, EMTF_ALGO)(const emtf
     return (int)hipGetLastError();
 }
 
-#if EMTF_NT == 1 && EMTF_ALGO == 0
-extern "C" int emtf_lds_floats(int D, int pop, int nt) { return emtf::lds_floats(D, pop, nt); }
+#if EMTF_KB == 2 && EMTF_ALGO == 0
+extern "C" int emtf_lds_floats_kb(int pop, int kb) { return emtf::lds_floats_kb(pop, kb); }
 #endif
