@@ -1,0 +1,89 @@
+"""C ABI checks that need no GPU: exported symbols and argument validation.
+
+emtf_create validates every descriptor before touching the device, so the
+error paths run on a host without a GPU.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:int|const char\*|void)\s+\**\s*(emtf_\w+)\s*\(", src, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from emtf import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert {"emtf_create", "emtf_run", "emtf_destroy", "emtf_get_population"} <= names
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+
+
+def _desc(func=0, dim=10, pop=32, xform=0, lo=-1.0, hi=1.0, rot=None, diag=None):
+    from emtf import _lib
+    f32p = _lib.f32p
+    shift = np.zeros(dim, np.float32)
+    keep = [shift, rot, diag]
+    d = _lib.TaskDesc(func, dim, pop, xform, lo, hi,
+                      rot.ctypes.data_as(f32p) if rot is not None else None, shift.ctypes.data_as(f32p),
+                      diag.ctypes.data_as(f32p) if diag is not None else None)
+    return d, keep
+
+
+def _create(descs, **kw):
+    from emtf import _lib
+    from emtf.engine import DEFAULT_PARAMS
+    p = dict(DEFAULT_PARAMS)
+    p.update(kw)
+    arr = (_lib.TaskDesc * len(descs))(*descs)
+    h = ctypes.c_void_p()
+    rc = _lib.lib().emtf_create(arr, len(descs), ctypes.byref(_lib.Params(**p)), ctypes.byref(h))
+    return rc, _lib.lib().emtf_last_error().decode()
+
+
+@pytest.mark.parametrize("kw, expect, msg", [
+    (dict(func=9), -1, "unknown func"),
+    (dict(dim=0), -1, "dim"),
+    (dict(dim=129), -4, "dim 129"),
+    (dict(pop=33), -1, "multiple of n_islands"),
+    (dict(lo=1.0, hi=1.0), -1, "bad box"),
+    (dict(xform=1), -1, "rotation matrix missing"),
+    (dict(xform=2), -1, "diagonal missing"),
+    (dict(pop=4096), -4, "pop 4096"),
+])
+def test_invalid_task_descriptors(kw, expect, msg):
+    extra = {}
+    if "pop" in kw and kw["pop"] == 33:
+        extra = dict(n_islands=2)
+    d, keep = _desc(**kw)
+    rc, err = _create([d], **extra)
+    assert rc == expect and msg in err, (rc, err)
+
+
+def test_invalid_params():
+    d, keep = _desc()
+    assert _create([d], algo=7)[0] == -1
+    assert "transfer_count" in _create([d, d], transfer_interval=5, transfer_count=20)[1]
+    assert "migrate_count" in _create([d], n_islands=4, migrate_interval=1, migrate_count=5)[1]
+    d2, keep2 = _desc(pop=30)
+    assert "even island" in _create([d2], algo=2, n_islands=2)[1]
+
+
+def test_null_arguments():
+    from emtf import _lib
+    L = _lib.lib()
+    assert L.emtf_create(None, 1, None, None) == -1
+    assert L.emtf_run(None, 5, None) == -1
+    assert L.emtf_destroy(None) == 0
