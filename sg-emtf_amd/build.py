@@ -29,10 +29,12 @@ def _run(cmd):
     return r.returncode, "\n".join(msgs)
 
 
-def build(verbose=False, jobs=None):
+def build(verbose=False, jobs=None, out=None, extra=()):
+    """Compile every variant and link LIB (or `out`); `extra` adds compiler flags."""
+    out = out or LIB
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 1)
-    units = [([HIPCC] + COMMON + [f"-DEMTF_KB={kb}", f"-DEMTF_ALGO={al}", "-c",
+    units = [([HIPCC] + COMMON + list(extra) + [f"-DEMTF_KB={kb}", f"-DEMTF_ALGO={al}", "-c",
                                   "-o", os.path.join(OBJ, f"island_kb{kb}_a{al}.o"), os.path.join(CSRC, "island.hip")])
              for kb, al in VARIANTS]
     units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, "emtf_host.o"), os.path.join(CSRC, "emtf_host.cpp")])
@@ -45,12 +47,12 @@ def build(verbose=False, jobs=None):
         if verbose and msg:
             print(msg)
     objs = [os.path.join(OBJ, f"island_kb{kb}_a{al}.o") for kb, al in VARIANTS] + [os.path.join(OBJ, "emtf_host.o")]
-    rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
+    rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
     if rc != 0:
         sys.stderr.write(msg + "\n")
         raise RuntimeError("link of libemtf.so failed")
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -190,6 +190,10 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     p.algo = prm->algo;
     p.F = prm->F;
     p.CR = prm->CR;
+    {
+        const double c = std::floor((double)prm->CR * 65536.0);
+        p.cr_threshold = (uint32_t)(c < 0.0 ? 0.0 : (c > 65536.0 ? 65536.0 : c));
+    }
     p.eta_c = prm->eta_c;
     p.eta_m = prm->eta_m;
     p.pm = prm->pm;

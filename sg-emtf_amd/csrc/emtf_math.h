@@ -212,19 +212,42 @@ __device__ __forceinline__ float weier_term(float z) {
     uint64_t y = m << (64 - s);
 #pragma unroll
     for (int k = 0; k <= 20; ++k) {
-        const uint32_t q = (uint32_t)(y >> 32);
+        // keep the conversion 32-bit: without the barrier the backend widens it to u64->f32
+        uint32_t q = (uint32_t)(y >> 32);
+        asm("" : "+v"(q));
         const float r = (float)q * 2.3283064365386963e-10f;
         const float sp = sinpi_half(__builtin_fminf(r, 1.0f - r));
         acc = fmaf(pow2i(1 - k), sp * sp, acc);
-        y = y + (y << 1);
+        // y <- 3 y mod 2^64 in one VALU op (the backend otherwise emits two 32-bit
+        // multiplies and register moves per step)
+        asm("v_lshl_add_u64 %0, %1, 1, %1" : "=v"(y) : "v"(y));
     }
     return acc;
 }
 
-// 16-bit uniform in (0,1]: half 0 = high 16 bits of w, half 1 = low 16 bits
-__device__ __forceinline__ float uniform16(uint32_t w, int half) {
-    const uint32_t v = half ? (w & 0xFFFFu) : (w >> 16);
-    return (float)(v + 1u) * 1.52587890625e-05f;
-}
+// 16-bit uniform u = (v + 1) 2^-16 in (0,1]: half 0 = high 16 bits of w, half 1 = low.
+// u > CR  <=>  v >= floor(CR 2^16) for every binary32 CR in [0,1] (both sides exact), so
+// the crossover test compares the raw 16-bit value with a precomputed threshold.
+__device__ __forceinline__ uint32_t bits16(uint32_t w, int half) { return half ? (w & 0xFFFFu) : (w >> 16); }
+
+// n mod d for a divisor fixed per task: Granlund-Montgomery round-up multiplier,
+// exact for every 32-bit n (d >= 1).
+struct FastMod {
+    uint32_t d, mul;
+    int sh;  // -1: d == 1
+    __device__ __forceinline__ void init(uint32_t dv) {
+        d = dv;
+        if (dv <= 1) { mul = 0; sh = -1; return; }
+        const int l = 32 - __builtin_clz(dv - 1);                     // ceil(log2 d)
+        mul = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - dv)) / dv + 1);
+        sh = l;
+    }
+    __device__ __forceinline__ uint32_t mod(uint32_t n) const {
+        if (sh < 0) return 0u;
+        const uint32_t t1 = __umulhi(n, mul);
+        const uint32_t q = (t1 + ((n - t1) >> 1)) >> (sh - 1);
+        return n - q * d;
+    }
+};
 
 }  // namespace emtf

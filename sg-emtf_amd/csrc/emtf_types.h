@@ -30,7 +30,7 @@ struct EngineParams {
     int32_t transfer_interval, transfer_count;
     float transfer_cr;
     uint32_t seed_lo, seed_hi;
-    int32_t rsv;
+    uint32_t cr_threshold;  // floor(CR 2^16) clamped to [0, 65536]: keep the parent iff u16 >= it
 };
 
 struct EpochArgs {

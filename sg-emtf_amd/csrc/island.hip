@@ -46,6 +46,7 @@ struct Shape {
 
 struct Geo {
     int D, Dh, pop, popP, ntiles, m;
+    FastMod modD, modM;  // jrand = w mod D, island-local index = w mod m
 };
 
 struct Lds {
@@ -266,22 +267,30 @@ __device__ __forceinline__ void trial_de(float (&tr)[4 * KB], const EpochArgs& a
     const int m = g.m, il = ie % m, base = ie - il;
     const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
     const U4 b0 = rng.block((uint32_t)gen, P_DE_IDX, (uint32_t)ie, 0u, 0u);
-    const int jrand = (int)(b0.w % (uint32_t)g.D);
-    IdxStream st{b0.x, b0.y, b0.z, 0u, 0, 3, 0};
-    int ex[3];
-    int nex = 0;
-    if (rand1) ex[nex++] = il;
-    const int r1 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
-    ex[nex++] = r1;
-    const int r2 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
-    ex[nex++] = r2;
-    const int r3 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+    const int jrand = (int)g.modD.mod(b0.w);
+    // Fast path: the first three words already give admissible mates (no rejection);
+    // otherwise replay the sequential rejection draw, which then consumes more words.
+    int r1 = (int)g.modM.mod(b0.x), r2 = (int)g.modM.mod(b0.y), r3 = (int)g.modM.mod(b0.z);
+    const bool ok = rand1 ? (m >= 4 && r1 != il && r2 != il && r2 != r1 && r3 != il && r3 != r1 && r3 != r2)
+                          : (m >= 3 && r2 != r1 && r3 != r1 && r3 != r2);
+    if (!ok) {
+        IdxStream st{b0.x, b0.y, b0.z, 0u, 0, 3, 0};
+        int ex[3];
+        int nex = 0;
+        if (rand1) ex[nex++] = il;
+        r1 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+        ex[nex++] = r1;
+        r2 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+        ex[nex++] = r2;
+        r3 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+    }
     const int off = h * S::HS;
     const float* xb = L.X + (base + (rand1 ? r1 : il)) * S::RS + off;
     const float* x2 = L.X + (base + r2) * S::RS + off;
     const float* x3 = L.X + (base + r3) * S::RS + off;
     const float* xi = L.X + ie * S::RS + off;
-    const float F = a.p.F, CR = a.p.CR, lo = td.lo, hi = td.hi;
+    const float F = a.p.F, lo = td.lo, hi = td.hi;
+    const uint32_t tcr = a.p.cr_threshold;
     U4 cb{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int sb = 0; sb < KB; ++sb) {
@@ -295,8 +304,8 @@ __device__ __forceinline__ void trial_de(float (&tr)[4 * KB], const EpochArgs& a
             const float sc = F * diff;
             float v = f4(vb, e) + sc;
             v = clampf(v, lo, hi);
-            const float u = uniform16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
-            const float t = (u > CR && d != jrand) ? f4(vi, e) : v;
+            const uint32_t u16 = bits16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
+            const float t = (u16 >= tcr && d != jrand) ? f4(vi, e) : v;
             tr[s] = (d < g.D) ? t : 0.0f;
         }
     }
@@ -308,8 +317,8 @@ __device__ __forceinline__ void trial_ga(float (&tr)[4 * KB], const EpochArgs& a
     using S = Shape<KB>;
     const int m = g.m, il = ie % m, base = ie - il;
     const U4 mb = rng.block((uint32_t)gen, P_GA_MATE, (uint32_t)(ie & ~1), 0u, 0u);
-    const int ra = (int)(pick(mb, (uint32_t)(il & 1)) % (uint32_t)m);
-    const int rb = (int)(pick(mb, (uint32_t)((il & 1) ^ 1)) % (uint32_t)m);
+    const int ra = (int)g.modM.mod(pick(mb, (uint32_t)(il & 1)));
+    const int rb = (int)g.modM.mod(pick(mb, (uint32_t)((il & 1) ^ 1)));
     const int off = h * S::HS;
     const float* xa = L.X + (base + ra) * S::RS + off;
     const float* xb = L.X + (base + rb) * S::RS + off;
@@ -594,6 +603,8 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
     g.popP = (g.pop + 31) & ~31;
     g.ntiles = g.popP / 32;
     g.m = g.pop / (a.p.n_islands > 0 ? a.p.n_islands : 1);
+    g.modD.init((uint32_t)g.D);
+    g.modM.init((uint32_t)g.m);
     Lds L;
     L.X = smem;
     L.Q = L.X + g.popP * RS;
@@ -678,8 +689,20 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
 
 // One workgroup per task; `order` lists the tasks longest-first so the
 // expensive objectives are dispatched before the cheap ones fill the gaps.
+// Register budget: with KB <= 8 (dims <= 64) three 4-wave workgroups fit in LDS per
+// CU, so the allocator is asked for 3 waves per SIMD (<= 168 VGPR+AGPR); larger
+// rows are LDS-bound to fewer workgroups and keep the default budget.
+#if !defined(EMTF_WAVES_PER_EU) && defined(EMTF_KB) && EMTF_KB <= 8
+#define EMTF_WAVES_PER_EU 3
+#endif
+#ifdef EMTF_WAVES_PER_EU
+#define EMTF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EMTF_WAVES_PER_EU, EMTF_WAVES_PER_EU)))
+#else
+#define EMTF_OCCUPANCY
+#endif
+
 template <int KB, int ALGO>
-__global__ __launch_bounds__(kThreads) void island_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
+__global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
     extern __shared__ float smem[];
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];

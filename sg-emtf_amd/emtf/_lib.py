@@ -7,7 +7,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libemtf.so")
+LIB_PATH = os.environ.get("EMTF_LIB") or os.path.join(os.path.dirname(_HERE), "libemtf.so")
 
 EMTF_OK = 0
 ERRORS = {-1: "EMTF_E_INVALID", -2: "EMTF_E_DEVICE", -3: "EMTF_E_NOMEM", -4: "EMTF_E_UNSUPPORTED"}
