@@ -112,6 +112,11 @@ def main():
     best, _ = eng.best()
     eng.close()
 
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "island_epoch_traffic.json")
+    if not a.only_func and (a.tasks, a.dim, a.pop, a.interval) == (1024, 50, 128, 20) and os.path.exists(tp):
+        traffic = json.load(open(tp))["hbm_bytes_per_launch"]
+
     cpu = None
     if a.cpu_gens > 0:
         cpu = cpu_baseline(tasks, params, a.cpu_gens, min(a.cpu_tasks, a.tasks))
@@ -126,7 +131,7 @@ def main():
                    "tasks": a.tasks, "dim": a.dim, "pop": a.pop, "generations": a.interval * a.steps,
                    "generations_per_step": a.interval, "parallelism": "tasks (1 GPU)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": traffic,
                      "algorithmic": "2*pop*D^2 FLOP per task-generation (rotation GEMM)",
                      "avg_launch_ms": gpu_ms / a.steps},
         "cpu_baseline": cpu,
