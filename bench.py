@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-gens", type=int, default=100, help="generations of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
+    ap.add_argument("--workload", default="plain", choices=["plain", "cvrp"],
+                    help="plain: the rotated continuous workload (default); cvrp: the CVRP GA workload")
     ap.add_argument("--only-func", default=None, help="give every task this objective (per-objective timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters)")
@@ -70,10 +72,59 @@ def cpu_baseline(tasks, params, gens, n_tasks):
                       f"{threads} threads, {dt:.2f} s)"}
 
 
+def bench_cvrp(a):
+    """CVRP GA workload: 512 instances, n ~ U{50..200} customers on integer U[0,100]^2, depot (50,50),
+    demand U{1..10}, capacity 50, pop 128, 4 islands with 2 elites every 25 generations; a step is
+    100 generations (one launch), 50 steps = 5000 generations."""
+    from emtf import hip
+    from emtf.cvrp import CvrpEngine, random_instances
+    n_inst = 512
+    steps = a.steps if a.steps != 100 else 50
+    inst = random_instances(n_inst, seed=a.seed)
+    params = dict(p_swap=0.3, p_2opt=0.3, n_islands=4, migrate_interval=25, migrate_count=2, seed=a.seed)
+    eng = CvrpEngine(inst, **params)
+    for _ in range(a.warmup):
+        eng.run(100)
+    hip.device_synchronize()
+    evs = [hip.Event() for _ in range(steps + 1)]
+    evs[0].record()
+    for k in range(steps):
+        eng.run(100)
+        evs[k + 1].record()
+    hip.device_synchronize()
+    ms = sum(evs[k].elapsed_ms(evs[k + 1]) for k in range(steps))
+    value = n_inst * 100 * steps / (ms / 1e3)
+    best, _ = eng.best()
+    cpu = None
+    if a.cpu_gens > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        o = oracle.CvrpEngine(inst, **params)
+        g = max(1, a.cpu_gens // 4)
+        t0 = time.perf_counter()
+        o.run(g, threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": n_inst * g / dt, "unit": "instance-generations/s", "cores": threads, "kind": "port",
+               "sample": f"{n_inst} instances x {g} generations (oracle, C/OpenMP, {threads} threads, {dt:.2f} s)"}
+    out = {"metric": "CVRP instance-generations/s @1 GPU", "value": value, "unit": "instance-generations/s",
+           "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+           "config": {"workload": "CVRP GA: 512 instances, n~U{50..200}, capacity 50, pop 128, 4 islands, "
+                                  "2 elites every 25 gens", "generations": 100 * steps, "generations_per_step": 100},
+           "cpu_baseline": cpu, "mean_best_cost": float(best.mean())}
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
+        return
+    if a.workload == "cvrp":
+        bench_cvrp(a)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine shards tasks across "

@@ -96,6 +96,38 @@ int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_strid
 
 int emtf_destroy(emtf_handle* h);
 
+/* ---- CVRP genetic algorithm (exact integers) -----------------------------
+ * Replaces the CVRP variant of the engine (proposed model/SG-EMTF_cvrp.cu: its
+ * pop_init :155-187, para_init :210-225, CrossOver :668-738, evaluation
+ * :859-962 and iter() :1495-1500).  Genome: a giant tour (uint8 customer ids
+ * 1..n); decode: Prins split; operators: OX crossover, swap / 2-opt mutation;
+ * distances nint(sqrt(dx^2+dy^2)) of integer coordinates. */
+typedef struct emtf_cvrp_instance {
+    int32_t n;               /* customers, 2..255 (depot is node 0) */
+    int32_t capacity;        /* vehicle capacity, >= every demand, < 64 x smallest demand */
+    int32_t pop;             /* individuals, <= 256, a multiple of n_islands */
+    const int32_t* xy;       /* (n+1) x 2 integer coordinates, row 0 = depot */
+    const int32_t* demand;   /* n+1 demands, demand[0] = 0, others in [1, 255] */
+} emtf_cvrp_instance;
+
+typedef struct emtf_cvrp_params {
+    float p_swap, p_2opt;    /* per child: swap mutation with p_swap, else 2-opt with p_2opt */
+    int32_t n_islands, migrate_interval, migrate_count;
+    uint64_t seed;
+} emtf_cvrp_params;
+
+typedef struct emtf_cvrp_handle emtf_cvrp_handle;
+
+int emtf_cvrp_create(const emtf_cvrp_instance* instances, int32_t n_instances, const emtf_cvrp_params* params,
+                     emtf_cvrp_handle** out);
+int emtf_cvrp_run(emtf_cvrp_handle* h, int32_t generations, void* stream);
+int emtf_cvrp_sync(emtf_cvrp_handle* h);
+/* tours[pop][n] and integer costs[pop] of one instance (synchronises) */
+int emtf_cvrp_get_population(emtf_cvrp_handle* h, int32_t instance, uint8_t* tours, int32_t* costs);
+/* best cost of every instance and its tour (rows of `stride` >= n) */
+int emtf_cvrp_get_best(emtf_cvrp_handle* h, int32_t* costs, uint8_t* tours, int32_t stride);
+int emtf_cvrp_destroy(emtf_cvrp_handle* h);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char* emtf_last_error(void);
 

@@ -48,7 +48,7 @@ enum { OR_XFORM_SHIFT = 0, OR_XFORM_ROTATE = 1, OR_XFORM_DIAG = 2 };
 enum { OR_ALGO_DE_RAND_1_BIN = 0, OR_ALGO_DE_CURRENT_1_BIN = 1, OR_ALGO_GA_SBX_PM = 2 };
 enum {
     OR_P_INIT = 1, OR_P_DE_IDX = 2, OR_P_DE_CR = 3, OR_P_GA_MATE = 4, OR_P_GA_SBX = 5,
-    OR_P_GA_PM = 6, OR_P_MIG = 7, OR_P_XFER = 8, OR_P_ISLX = 9
+    OR_P_GA_PM = 6, OR_P_MIG = 7, OR_P_XFER = 8, OR_P_ISLX = 9, OR_P_CV_INIT = 16, OR_P_CV_GEN = 17
 };
 
 /* One continuous task.  z = Q (x - o) (ROTATE), z = x - o (SHIFT),
@@ -111,6 +111,31 @@ int32_t or_generation(const or_engine* e);
 void or_get_population(const or_engine* e, int32_t task, float* x, float* fit);
 void or_set_population(or_engine* e, int32_t task, const float* x, const float* fit);
 void or_get_best(const or_engine* e, float* fit, float* x_best, int32_t x_stride);
+
+/* ---- CVRP (exact integers) -------------------------------------------------- */
+typedef struct {
+    int32_t n;               /* customers 1..n, depot 0; n <= 255 */
+    int32_t capacity;
+    int32_t pop;
+    const int32_t* xy;       /* (n+1) x 2 integer coordinates, row 0 = depot */
+    const int32_t* demand;   /* n+1, demand[0] = 0, each >= 1 */
+} or_cvrp_instance;
+
+typedef struct {
+    float p_swap, p_2opt;    /* mutation: swap with p_swap, else 2-opt with p_2opt */
+    int32_t n_islands, migrate_interval, migrate_count;
+    uint64_t seed;
+} or_cvrp_params;
+
+typedef struct or_cvrp or_cvrp;
+int32_t or_cvrp_dist(int32_t x0, int32_t y0, int32_t x1, int32_t y1);
+int32_t or_cvrp_split(const uint8_t* tour, int n, const uint8_t* dist, const uint8_t* dem, int cap);
+void or_cvrp_child(const uint8_t* p1, const uint8_t* p2, int n, int a, int b, int mut, int c, int d,
+                   uint8_t* child);
+or_cvrp* or_cvrp_create(const or_cvrp_instance* inst, int32_t n_inst, const or_cvrp_params* p);
+void or_cvrp_run(or_cvrp* e, int32_t gens, int32_t n_threads);
+void or_cvrp_get(const or_cvrp* e, int32_t inst, uint8_t* tours, int32_t* cost);
+void or_cvrp_destroy(or_cvrp* e);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,16 @@ class OrParams(ctypes.Structure):
                 ("transfer_cr", ctypes.c_float), ("seed", ctypes.c_uint64)]
 
 
+class CvrpInstance(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("capacity", ctypes.c_int32), ("pop", ctypes.c_int32),
+                ("xy", ctypes.POINTER(ctypes.c_int32)), ("demand", ctypes.POINTER(ctypes.c_int32))]
+
+
+class CvrpParams(ctypes.Structure):
+    _fields_ = [("p_swap", ctypes.c_float), ("p_2opt", ctypes.c_float), ("n_islands", ctypes.c_int32),
+                ("migrate_interval", ctypes.c_int32), ("migrate_count", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+
 def build(force=False):
     if force or not os.path.exists(_LIB):
         subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
@@ -70,6 +80,18 @@ def lib():
         L.or_get_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
         L.or_set_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
         L.or_get_best.argtypes = [ctypes.c_void_p, f32p, f32p, ctypes.c_int32]
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.or_cvrp_dist.argtypes = [ctypes.c_int32] * 4
+        L.or_cvrp_dist.restype = ctypes.c_int32
+        L.or_cvrp_split.argtypes = [u8p, ctypes.c_int, u8p, u8p, ctypes.c_int]
+        L.or_cvrp_split.restype = ctypes.c_int32
+        L.or_cvrp_child.argtypes = [u8p, u8p] + [ctypes.c_int] * 6 + [u8p]
+        L.or_cvrp_create.argtypes = [ctypes.POINTER(CvrpInstance), ctypes.c_int32, ctypes.POINTER(CvrpParams)]
+        L.or_cvrp_create.restype = ctypes.c_void_p
+        L.or_cvrp_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+        L.or_cvrp_get.argtypes = [ctypes.c_void_p, ctypes.c_int32, u8p, i32p]
+        L.or_cvrp_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -196,6 +218,76 @@ class Engine:
     def close(self):
         if self._h:
             lib().or_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------- CVRP
+def _u8(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def _i32(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def cvrp_dist_matrix(xy):
+    n1 = xy.shape[0]
+    d = np.zeros((n1, n1), np.uint8)
+    for i in range(n1):
+        for j in range(n1):
+            d[i, j] = lib().or_cvrp_dist(int(xy[i, 0]), int(xy[i, 1]), int(xy[j, 0]), int(xy[j, 1]))
+    return d
+
+
+def cvrp_split(tour, dist, demand, cap):
+    t = np.ascontiguousarray(tour, np.uint8)
+    return lib().or_cvrp_split(_u8(t), len(t), _u8(np.ascontiguousarray(dist, np.uint8)),
+                               _u8(np.ascontiguousarray(demand, np.uint8)), int(cap))
+
+
+def cvrp_child(p1, p2, a, b, mut, c, d):
+    p1 = np.ascontiguousarray(p1, np.uint8)
+    p2 = np.ascontiguousarray(p2, np.uint8)
+    out = np.zeros_like(p1)
+    lib().or_cvrp_child(_u8(p1), _u8(p2), len(p1), a, b, mut, c, d, _u8(out))
+    return out
+
+
+class CvrpEngine:
+    """The oracle's CVRP GA over a list of instance dicts (n, capacity, pop, xy, demand)."""
+
+    def __init__(self, instances, p_swap=0.3, p_2opt=0.3, n_islands=1, migrate_interval=0, migrate_count=0, seed=0):
+        self._keep = []
+        arr = []
+        for ins in instances:
+            xy = np.ascontiguousarray(ins["xy"], np.int32)
+            dem = np.ascontiguousarray(ins["demand"], np.int32)
+            self._keep += [xy, dem]
+            arr.append(CvrpInstance(int(ins["n"]), int(ins["capacity"]), int(ins["pop"]), _i32(xy), _i32(dem)))
+        carr = (CvrpInstance * len(arr))(*arr)
+        self._p = CvrpParams(p_swap, p_2opt, n_islands, migrate_interval, migrate_count, seed)
+        self._h = lib().or_cvrp_create(carr, len(arr), ctypes.byref(self._p))
+        self.instances = instances
+
+    def run(self, gens, threads=0):
+        lib().or_cvrp_run(self._h, int(gens), int(threads))
+
+    def population(self, i):
+        ins = self.instances[i]
+        t = np.zeros((ins["pop"], ins["n"]), np.uint8)
+        c = np.zeros(ins["pop"], np.int32)
+        lib().or_cvrp_get(self._h, i, _u8(t), _i32(c))
+        return t, c
+
+    def close(self):
+        if self._h:
+            lib().or_cvrp_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -37,7 +37,8 @@ def build(verbose=False, jobs=None, out=None, extra=()):
     units = [([HIPCC] + COMMON + list(extra) + [f"-DEMTF_KB={kb}", f"-DEMTF_ALGO={al}", "-c",
                                   "-o", os.path.join(OBJ, f"island_kb{kb}_a{al}.o"), os.path.join(CSRC, "island.hip")])
              for kb, al in VARIANTS]
-    units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, "emtf_host.o"), os.path.join(CSRC, "emtf_host.cpp")])
+    for name in ("emtf_host.cpp", "cvrp_host.cpp", "cvrp.hip"):
+        units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, name.split(".")[0] + ".o"), os.path.join(CSRC, name)])
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(_run, units))
     for (rc, msg), cmd in zip(results, units):
@@ -46,7 +47,8 @@ def build(verbose=False, jobs=None, out=None, extra=()):
             raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
         if verbose and msg:
             print(msg)
-    objs = [os.path.join(OBJ, f"island_kb{kb}_a{al}.o") for kb, al in VARIANTS] + [os.path.join(OBJ, "emtf_host.o")]
+    objs = [os.path.join(OBJ, f"island_kb{kb}_a{al}.o") for kb, al in VARIANTS] + \
+        [os.path.join(OBJ, n + ".o") for n in ("emtf_host", "cvrp_host", "cvrp")]
     rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
     if rc != 0:
         sys.stderr.write(msg + "\n")

@@ -150,6 +150,9 @@ extern "C" {
 
 const char* emtf_last_error(void) { return g_err.c_str(); }
 
+// shared with the CVRP runtime (cvrp_host.cpp); not part of the public header
+void emtf_set_error(const char* msg) { g_err = msg ? msg : ""; }
+
 int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params* prm, emtf_handle** out) {
     g_err.clear();
     if (!out) return fail(EMTF_E_INVALID, "out is NULL");
