@@ -96,6 +96,18 @@ int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_strid
 
 int emtf_destroy(emtf_handle* h);
 
+/* ---- Objective and adjoint (device pointers, asynchronous on `stream`) -------
+ * f[i] = f(Q (x_i - o)) for n points x[n][dim], bit for bit the engine's evaluator
+ * (rot NULL: identity; shift NULL: zero).  Replaces the reference's evaluation()
+ * (proposed model/SG-EMTF_DE.cu:834-1069) as a reusable operator. */
+int emtf_objective_fwd(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
+                       float* f, void* stream);
+/* gx[i] = gf[i] * Q^T (df/dz)(Q (x_i - o)): the hand-written adjoint in binary32.
+ * No counterpart in the reference (it has no gradients); used by the gradient polish
+ * and by the torch.autograd.Function wrapper. */
+int emtf_objective_adjoint(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot,
+                           const float* shift, const float* gf, float* gx, void* stream);
+
 /* ---- CVRP genetic algorithm (exact integers) -----------------------------
  * Replaces the CVRP variant of the engine (proposed model/SG-EMTF_cvrp.cu: its
  * pop_init :155-187, para_init :210-225, CrossOver :668-738, evaluation

@@ -245,7 +245,7 @@ static void dfdz64(int func, int D, const double* z, double* g) {
     case OR_FUNC_ACKLEY: {
         for (int j = 0; j < D; ++j) { s += z[j] * z[j]; c += cos(2.0 * M_PI * z[j]); }
         double r = sqrt(s / D);
-        double k1 = (r > 0.0) ? 2.0 * exp(-0.2 * r) / (r * D) : 0.0;
+        double k1 = (r > 0.0) ? 4.0 * exp(-0.2 * r) / (r * D) : 0.0;  /* 20 * 0.2 * dr/dz */
         double k2 = exp(c / D) * 2.0 * M_PI / D;
         for (int j = 0; j < D; ++j) g[j] = k1 * z[j] + k2 * sin(2.0 * M_PI * z[j]);
         break;

@@ -27,6 +27,8 @@
 EMTF_DECL_PAIR(2) EMTF_DECL_PAIR(4) EMTF_DECL_PAIR(7) EMTF_DECL_PAIR(8) EMTF_DECL_PAIR(12) EMTF_DECL_PAIR(13)
 EMTF_DECL_PAIR(16)
 extern "C" int emtf_lds_floats_kb(int pop, int kb);
+#define EMTF_DECL_OBJ(kb) extern "C" int emtf_launch_objective_kb##kb(const emtf::ObjArgs* a, int adjoint, hipStream_t s);
+EMTF_DECL_OBJ(2) EMTF_DECL_OBJ(4) EMTF_DECL_OBJ(7) EMTF_DECL_OBJ(8) EMTF_DECL_OBJ(12) EMTF_DECL_OBJ(13) EMTF_DECL_OBJ(16)
 
 typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
 
@@ -378,6 +380,50 @@ int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_strid
         if (x_best) std::memcpy(x_best + (size_t)t * x_stride, x.data() + td.x_off + (int64_t)best * td.dim, sizeof(float) * td.dim);
     }
     return EMTF_OK;
+}
+
+static int objective_call(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
+                          const float* gf, float* f, float* gx, void* stream, int adjoint) {
+    g_err.clear();
+    if (func < EMTF_SPHERE || func > EMTF_ELLIPTIC) return fail(EMTF_E_INVALID, "unknown func");
+    if (dim < 1) return fail(EMTF_E_INVALID, "dim < 1");
+    if (dim > 128) return fail(EMTF_E_UNSUPPORTED, "dim > 128 not built");
+    if (n < 0 || !x || (adjoint ? (!gf || !gx) : !f)) return fail(EMTF_E_INVALID, "NULL buffer or n < 0");
+    if (n == 0) return EMTF_OK;
+    emtf::ObjArgs a{};
+    a.func = func;
+    a.dim = dim;
+    a.n = n;
+    a.has_rot = rot != nullptr;
+    a.x = x;
+    a.rot = rot;
+    a.shift = shift;
+    a.gf = gf;
+    a.f = f;
+    a.gx = gx;
+    func_table(func, dim, a.tab);
+    int rc = -1;
+    switch (pick_kb(dim)) {
+    case 2: rc = emtf_launch_objective_kb2(&a, adjoint, (hipStream_t)stream); break;
+    case 4: rc = emtf_launch_objective_kb4(&a, adjoint, (hipStream_t)stream); break;
+    case 7: rc = emtf_launch_objective_kb7(&a, adjoint, (hipStream_t)stream); break;
+    case 8: rc = emtf_launch_objective_kb8(&a, adjoint, (hipStream_t)stream); break;
+    case 12: rc = emtf_launch_objective_kb12(&a, adjoint, (hipStream_t)stream); break;
+    case 13: rc = emtf_launch_objective_kb13(&a, adjoint, (hipStream_t)stream); break;
+    case 16: rc = emtf_launch_objective_kb16(&a, adjoint, (hipStream_t)stream); break;
+    }
+    if (rc != 0) return fail(EMTF_E_DEVICE, std::string("objective kernel launch: ") + hipGetErrorString((hipError_t)rc));
+    return EMTF_OK;
+}
+
+int emtf_objective_fwd(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
+                       float* f, void* stream) {
+    return objective_call(func, dim, n, x, rot, shift, nullptr, f, nullptr, stream, 0);
+}
+
+int emtf_objective_adjoint(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
+                           const float* gf, float* gx, void* stream) {
+    return objective_call(func, dim, n, x, rot, shift, gf, nullptr, gx, stream, 1);
 }
 
 int emtf_destroy(emtf_handle* h) {

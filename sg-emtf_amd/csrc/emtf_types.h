@@ -49,4 +49,16 @@ struct EpochArgs {
     EngineParams p;
 };
 
+// Stand-alone objective / adjoint evaluation of n points (device pointers).
+struct ObjArgs {
+    int32_t func, dim, n, has_rot;
+    const float* x;      // [n][dim]
+    const float* rot;    // [dim][dim] or null (identity)
+    const float* shift;  // [dim] or null (zero)
+    const float* gf;     // [n] upstream gradient (adjoint)
+    float* f;            // [n] objective (forward)
+    float* gx;           // [n][dim] gradient (adjoint)
+    float tab[128];      // per-dimension table (Elliptic weights / Griewank scales)
+};
+
 }  // namespace emtf

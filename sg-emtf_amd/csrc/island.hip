@@ -719,6 +719,215 @@ __global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(E
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Stand-alone objective and adjoint (emtf_objective_fwd / emtf_objective_adjoint).
+// A wave evaluates 32 points.  The forward pass is the engine's evaluator (bit
+// for bit).  The adjoint forms dz = df/dz per dimension in binary32 from z
+// staged in LDS, scales it by the upstream gradient, and contracts
+// gx = Q^T dz with a second run of v_mfma_f32_32x32x2_f32 (A = Q read
+// transposed from LDS, B = dz with one point per column).
+// ---------------------------------------------------------------------------
+template <int KB>
+struct ObjShape {
+    static constexpr int PB = (KB <= 8) ? 128 : 32;  // points per workgroup
+    static constexpr int ZS = 32 * Shape<KB>::NT + 1; // LDS stride of staged z rows
+};
+
+template <int KB>
+__host__ __device__ inline int obj_lds_floats() {
+    using S = Shape<KB>;
+    constexpr int PB = ObjShape<KB>::PB, ZS = ObjShape<KB>::ZS;
+    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * PB * ZS;
+}
+
+// d f / d z_j of the lane's point for j = 2 s + h (the B-operand layout of the
+// second contraction), from the full z row in LDS.
+template <int KB, int FUNC>
+__device__ __forceinline__ void dfdz(float (&g)[4 * KB], const float* z, float* pre, int D, const float* tab, int h) {
+    constexpr int HS = 4 * KB;
+    float s1 = 0.0f, s2 = 0.0f;
+    if constexpr (FUNC == F_ACKLEY) {
+        for (int j = 0; j < D; ++j) { s1 = s1 + z[j] * z[j]; s2 = s2 + sinpi_sq(z[j]); }
+    }
+    if constexpr (FUNC == F_GRIEWANK) {
+        // prefix products of cos(theta_k) with cos(theta) = 1 - 2 sin^2(theta/2)
+        float p = 1.0f;
+        for (int j = 0; j < D; ++j) {
+            pre[j] = p;
+            const float c = 1.0f - 2.0f * sinpi_sq(z[j] * tab[j]);
+            p = p * c;
+        }
+    }
+    const float Df = (float)D;
+    float ka = 0.0f, kb = 0.0f;
+    if constexpr (FUNC == F_ACKLEY) {
+        const float r = sqrtf(s1 / Df);
+        const float e1 = expm1_f(-0.2f * r) + 1.0f;
+        const float e2 = expm1_f(-2.0f * (s2 / Df)) + 1.0f;
+        ka = (r > 0.0f) ? (4.0f * e1) / (Df * r) : 0.0f;
+        kb = (6.28318548f * 2.71828175f / Df) * e2;
+    }
+    float suf = 1.0f;  // Griewank suffix product, walked downwards
+    int jn = D - 1;
+#pragma unroll
+    for (int s = HS - 1; s >= 0; --s) {
+        const int j = 2 * s + h;
+        float v = 0.0f;
+        if (j < D) {
+            const float zj = z[j];
+            if constexpr (FUNC == F_SPHERE) v = 2.0f * zj;
+            else if constexpr (FUNC == F_ELLIPTIC) v = 2.0f * tab[j] * zj;
+            else if constexpr (FUNC == F_RASTRIGIN) v = 2.0f * zj + 62.8318531f * sin2pi(zj);
+            else if constexpr (FUNC == F_ACKLEY) v = ka * zj + kb * sin2pi(zj);
+            else if constexpr (FUNC == F_ROSENBROCK) {
+                if (j + 1 < D) v = 400.0f * zj * (zj * zj - z[j + 1]) + 2.0f * (zj - 1.0f);
+                if (j > 0) v = v - 200.0f * (z[j - 1] * z[j - 1] - zj);
+            } else if constexpr (FUNC == F_GRIEWANK) {
+                while (jn > j) {  // suffix = prod_{k > j} cos(theta_k)
+                    suf = suf * (1.0f - 2.0f * sinpi_sq(z[jn] * tab[jn]));
+                    --jn;
+                }
+                const float sn = sin2pi(zj * tab[j]);  // sin(theta_j)
+                v = zj / 2000.0f + pre[j] * suf * sn * (6.28318548f * tab[j]);
+            } else if constexpr (FUNC == F_WEIERSTRASS) {
+                // sum_k 2^(1-k) pi 3^k sin(2 pi 3^k z), fractions of 3^k |z| as in the forward pass
+                const float a = __builtin_fabsf(zj);
+                float acc = 0.0f;
+                if (a < 8388608.0f) {
+                    float c = 6.28318548f;  // 2 pi (1.5)^k
+                    if (a < 2.32830644e-10f) {
+                        float p3 = 1.0f;
+                        for (int k = 0; k <= 20; ++k) {
+                            acc = fmaf(c, sin2pi(a * p3), acc);
+                            c = c * 1.5f;
+                            p3 = p3 * 3.0f;
+                        }
+                    } else {
+                        const uint32_t u = __float_as_uint(a);
+                        const int ex = (int)((u >> 23) & 0xFFu) - 127;
+                        const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
+                        uint64_t y = m << (64 - (23 - ex));
+                        for (int k = 0; k <= 20; ++k) {
+                            const float r = (float)(uint32_t)(y >> 32) * 2.3283064365386963e-10f;
+                            acc = fmaf(c, sin2pi(r), acc);
+                            c = c * 1.5f;
+                            y = y + (y << 1);
+                        }
+                    }
+                }
+                v = __builtin_copysignf(1.0f, zj) * acc;
+            } else if constexpr (FUNC == F_SCHWEFEL) {
+                const float sq = sqrtf(__builtin_fabsf(zj));
+                const float t = sq * 0.159154937f;
+                v = -(sin2pi(t) + 0.5f * sq * sin2pi(t + 0.25f));
+            }
+        }
+        g[s] = v;
+    }
+}
+
+template <int KB, int FUNC, bool ADJ>
+__global__ __launch_bounds__(2 * ObjShape<KB>::PB) void objective_kernel(ObjArgs a) {
+    using S = Shape<KB>;
+    constexpr int RS = S::RS, HS = S::HS, NT = S::NT, PB = ObjShape<KB>::PB, ZS = ObjShape<KB>::ZS;
+    extern __shared__ float smem[];
+    float* Q = smem;
+    float* o = Q + 32 * NT * RS;
+    float* tab = o + RS;
+    float* Zs = tab + 32 * NT;
+    float* Pre = Zs + PB * ZS;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int D = a.dim;
+    for (int e = tid; e < 32 * NT * RS + RS + 32 * NT; e += nthr) smem[e] = 0.0f;
+    __syncthreads();
+    for (int e = tid; e < D * D; e += nthr) {
+        const int j = e / D, d = e - j * D;
+        Q[j * RS + (d & 1) * HS + (d >> 1)] = a.has_rot ? a.rot[e] : (j == d ? 1.0f : 0.0f);
+    }
+    for (int d = tid; d < D; d += nthr) {
+        o[(d & 1) * HS + (d >> 1)] = a.shift ? a.shift[d] : 0.0f;
+        tab[d] = a.tab[d];
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
+    const int pt = blockIdx.x * PB + wave * 32 + col;
+    const bool valid = pt < a.n;
+    const float* xr = a.x + (int64_t)(valid ? pt : 0) * D;
+    float x[HS];
+#pragma unroll
+    for (int s = 0; s < HS; ++s) {
+        const int d = 2 * s + h;
+        x[s] = (d < D) ? xr[d] : 0.0f;
+    }
+    Geo g;
+    g.D = D;
+    Lds L;
+    L.Q = Q;
+    L.o = o;
+    L.tab = tab;
+    if constexpr (!ADJ) {
+        const float f = evaluate_tile<KB, FUNC>(x, g, L, col, h);
+        if (valid && h == 0) a.f[pt] = f;
+    } else {
+        // z = Q (x - o) as in the forward pass, staged row-wise in LDS
+        f16v acc[NT];
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+#pragma unroll
+        for (int sb = 0; sb < KB; ++sb) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int s = 4 * sb + e;
+                const float w = x[s] - o[h * HS + s];
+#pragma unroll
+                for (int jt = 0; jt < NT; ++jt)
+                    acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(Q[(32 * jt + col) * RS + h * HS + s], w, acc[jt], 0, 0, 0);
+            }
+        }
+        float* zrow = Zs + (wave * 32 + col) * ZS;
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) zrow[32 * jt + 8 * (r >> 2) + 4 * h + (r & 3)] = acc[jt][r];
+        __syncthreads();
+        float dz[HS];
+        dfdz<KB, FUNC>(dz, zrow, Pre + (wave * 32 + col) * ZS + (h ? 0 : 0), D, tab, h);
+        const float up = valid ? a.gf[pt] : 0.0f;
+#pragma unroll
+        for (int s = 0; s < HS; ++s) dz[s] = dz[s] * up;
+        // gx = Q^T dz: A[row d][k j] = Q[j][d]
+        f16v gacc[NT];
+#pragma unroll
+        for (int dt = 0; dt < NT; ++dt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) gacc[dt][r] = 0.0f;
+        const int dcol = col;  // lane's A row within the tile
+#pragma unroll
+        for (int s = 0; s < HS; ++s) {
+            const int j = 2 * s + h;
+#pragma unroll
+            for (int dt = 0; dt < NT; ++dt) {
+                const int d = 32 * dt + dcol;
+                const float qa = (j < 32 * NT) ? Q[j * RS + (d & 1) * HS + (d >> 1)] : 0.0f;
+                gacc[dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa, dz[s], gacc[dt], 0, 0, 0);
+            }
+        }
+        if (valid) {
+            float* gr = a.gx + (int64_t)pt * D;
+#pragma unroll
+            for (int dt = 0; dt < NT; ++dt)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int d = 32 * dt + 8 * (r >> 2) + 4 * h + (r & 3);
+                    if (d < D) gr[d] = gacc[dt][r];
+                }
+        }
+    }
+}
+
 }  // namespace emtf
 
 #if !defined(EMTF_KB) || !defined(EMTF_ALGO)
@@ -745,4 +954,41 @@ extern "C" int EMTF_NAME(emtf_launch_epoch_kb, EMTF_KB, a, EMTF_ALGO)(const emtf
 
 #if EMTF_KB == 2 && EMTF_ALGO == 0
 extern "C" int emtf_lds_floats_kb(int pop, int kb) { return emtf::lds_floats_kb(pop, kb); }
+#endif
+
+#if EMTF_ALGO == 0
+// Objective kernels live in the DE translation unit of each KB.
+template <int FUNC, bool ADJ>
+static hipError_t launch_obj(const emtf::ObjArgs& a, hipStream_t stream) {
+    constexpr int KB = EMTF_KB;
+    auto k = &emtf::objective_kernel<KB, FUNC, ADJ>;
+    const int bytes = 4 * emtf::obj_lds_floats<KB>();
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    constexpr int PB = emtf::ObjShape<KB>::PB;
+    hipLaunchKernelGGL(k, dim3((a.n + PB - 1) / PB), dim3(2 * PB), bytes, stream, a);
+    return hipGetLastError();
+}
+
+template <bool ADJ>
+static hipError_t launch_obj_any(const emtf::ObjArgs& a, hipStream_t s) {
+    switch (a.func) {
+    case 0: return launch_obj<0, ADJ>(a, s);
+    case 1: return launch_obj<1, ADJ>(a, s);
+    case 2: return launch_obj<2, ADJ>(a, s);
+    case 3: return launch_obj<3, ADJ>(a, s);
+    case 4: return launch_obj<4, ADJ>(a, s);
+    case 5: return launch_obj<5, ADJ>(a, s);
+    case 6: return launch_obj<6, ADJ>(a, s);
+    case 7: return launch_obj<7, ADJ>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+#define EMTF_OBJ_NAME2(a, b) a##b
+#define EMTF_OBJ_NAME(a, b) EMTF_OBJ_NAME2(a, b)
+extern "C" int EMTF_OBJ_NAME(emtf_launch_objective_kb, EMTF_KB)(const emtf::ObjArgs* a, int adjoint, hipStream_t s) {
+    return (int)(adjoint ? launch_obj_any<true>(*a, s) : launch_obj_any<false>(*a, s));
+}
 #endif
