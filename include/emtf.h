@@ -64,6 +64,9 @@ typedef struct emtf_params {
     int32_t transfer_count;     /* elites per transfer, <= pop / 2 */
     float transfer_cr;          /* keep own gene when u <= transfer_cr; 0 takes every donor gene */
     uint64_t seed;
+    int32_t polish_interval;    /* gradient polish period in generations (0 = off); no reference counterpart */
+    int32_t polish_steps;       /* Armijo steps per polished elite */
+    int32_t polish_topk;        /* elites polished per task */
 } emtf_params;
 
 typedef struct emtf_handle emtf_handle;

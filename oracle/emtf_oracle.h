@@ -73,6 +73,9 @@ typedef struct {
     int32_t transfer_count;
     float transfer_cr;            /* keep own gene if u <= transfer_cr (0 = take all) */
     uint64_t seed;
+    int32_t polish_interval;      /* gradient polish period (0 = off) */
+    int32_t polish_steps;         /* Armijo steps per elite */
+    int32_t polish_topk;          /* elites polished per task */
 } or_params;
 
 /* ---- Philox ------------------------------------------------------------ */
@@ -98,6 +101,8 @@ float or_finish32(const or_task* t, const float* z, const float* tab);
 void or_eval32(const or_task* t, int32_t n, const float* x, float* f);
 /* FP64 textbook evaluation (independent formulas, libm). */
 void or_eval64(const or_task* t, int32_t n, const double* x, double* f);
+/* FP32 gradient (contract arithmetic of the device adjoint). */
+void or_grad32(const or_task* t, int32_t n, const float* x, float* gx);
 /* FP64 gradient of the textbook objective at x[n][dim]. */
 void or_grad64(const or_task* t, int32_t n, const double* x, double* g);
 

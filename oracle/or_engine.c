@@ -362,6 +362,51 @@ static void island_migration(or_engine* e, int ti) {
     }
 }
 
+/* Gradient polish of the top-k elites: Armijo steps along -grad32 from a Polyak
+ * step f / |g|^2, halving up to 24 times; a point stops at its first failed step. */
+#define POLISH_MAX_HALVINGS 24
+static void polish_task(or_engine* e, int ti, float* work) {
+    task_state* s = &e->ts[ti];
+    const or_task* t = &s->t;
+    const int D = t->dim, P = t->pop;
+    const int K = e->p.polish_topk < P ? e->p.polish_topk : P;
+    order_of(s->fit, P, s->order);
+    float* x = s->snapX;          /* scratch rows: x, g, xn */
+    float* g = x + D;
+    float* xn = g + D;
+    for (int k = 0; k < K; ++k) {
+        const int i = s->order[k];
+        memcpy(x, s->X + (size_t)i * D, sizeof(float) * (size_t)D);
+        float f = s->fit[i];
+        for (int step = 0; step < e->p.polish_steps; ++step) {
+            or_grad32(t, 1, x, g);
+            float g2 = 0.0f;
+            for (int d = 0; d < D; ++d) g2 = fmaf(g[d], g[d], g2);
+            if (!(g2 > 0.0f) || !(f > 0.0f)) break;
+            float alpha = f / g2;
+            int accepted = 0;
+            float fn = 0.0f;
+            for (int b = 0; b < POLISH_MAX_HALVINGS; ++b) {
+                for (int d = 0; d < D; ++d) {
+                    float st = alpha * g[d];
+                    xn[d] = clampf(x[d] - st, t->lo, t->hi);
+                }
+                eval_points(s, 1, xn, &fn, work);
+                float c1 = 1.0e-4f * alpha;
+                float c2 = c1 * g2;
+                float thr = f - c2;
+                if (fn <= thr) { accepted = 1; break; }
+                alpha = alpha * 0.5f;
+            }
+            if (!accepted) break;
+            memcpy(x, xn, sizeof(float) * (size_t)D);
+            f = fn;
+        }
+        memcpy(s->X + (size_t)i * D, x, sizeof(float) * (size_t)D);
+        s->fit[i] = f;
+    }
+}
+
 static void generation(or_engine* e, int ti, int gen, float* work) {
     task_state* s = &e->ts[ti];
     const or_params* p = &e->p;
@@ -373,6 +418,8 @@ static void generation(or_engine* e, int ti, int gen, float* work) {
     select_task(s);
     if (p->n_islands > 1 && p->migrate_interval > 0 && p->migrate_count > 0 && gen % p->migrate_interval == 0)
         island_migration(e, ti);
+    if (p->polish_interval > 0 && p->polish_steps > 0 && p->polish_topk > 0 && gen % p->polish_interval == 0)
+        polish_task(e, ti, work);
 }
 
 /* ---- inter-task transfer -------------------------------------------------- */
@@ -466,7 +513,7 @@ or_engine* or_create(const or_task* tasks, int32_t n_tasks, const or_params* p) 
         int kk = P > e->p.transfer_count ? P : e->p.transfer_count;
         s->trial = dupf(NULL, (size_t)kk * D);
         s->tfit = dupf(NULL, (size_t)kk);
-        s->snapX = dupf(NULL, (size_t)P * D);
+        s->snapX = dupf(NULL, (size_t)(P > 3 ? P : 3) * D);
         s->snapF = dupf(NULL, (size_t)P);
         s->order = (int*)calloc((size_t)P, sizeof(int));
         s->elites = dupf(NULL, (size_t)(e->p.transfer_count > 0 ? e->p.transfer_count : 1) * D);

@@ -300,3 +300,160 @@ void or_grad64(const or_task* t, int32_t n, const double* x, double* g) {
     }
     free(z);
 }
+
+
+/* ---- FP32 gradient (contract arithmetic, mirrors the device adjoint) ------ */
+/* d f / d z_j in binary32 from z (sequential sums and products over j). */
+static void dfdz32(int func, int D, const float* z, const float* tab, float* g) {
+    float s1 = 0.0f, s2 = 0.0f;
+    float* pre = NULL;
+    if (func == OR_FUNC_ACKLEY)
+        for (int j = 0; j < D; ++j) { float v = z[j] * z[j]; s1 = s1 + v; s2 = s2 + or_sinpi_sq(z[j]); }
+    if (func == OR_FUNC_GRIEWANK) {
+        pre = (float*)malloc(sizeof(float) * (size_t)D);
+        float p = 1.0f;
+        for (int j = 0; j < D; ++j) {
+            pre[j] = p;
+            float t = z[j] * tab[j];
+            float c = 1.0f - 2.0f * or_sinpi_sq(t);
+            p = p * c;
+        }
+    }
+    const float Df = (float)D;
+    float ka = 0.0f, kb = 0.0f;
+    if (func == OR_FUNC_ACKLEY) {
+        float q = s1 / Df;
+        float r = sqrtf(q);
+        float e1 = or_expm1(-0.2f * r) + 1.0f;
+        float c2 = s2 / Df;
+        float e2 = or_expm1(-2.0f * c2) + 1.0f;
+        float n1 = 4.0f * e1;
+        float d1 = Df * r;
+        ka = (r > 0.0f) ? n1 / d1 : 0.0f;
+        float kc = 6.28318548f * 2.71828175f;
+        kc = kc / Df;
+        kb = kc * e2;
+    }
+    float suf = 1.0f;
+    int jn = D - 1;
+    /* visit j in the device order: descending slots s, j = 2 s + h for h = 0, 1 */
+    for (int jj = D - 1; jj >= 0; --jj) g[jj] = 0.0f;
+    for (int h = 0; h < 2; ++h) {
+        suf = 1.0f;
+        jn = D - 1;
+        for (int s = (D + 1) / 2; s >= 0; --s) {
+            const int j = 2 * s + h;
+            if (j >= D) continue;
+            const float zj = z[j];
+            float v = 0.0f;
+            switch (func) {
+            case OR_FUNC_SPHERE: v = 2.0f * zj; break;
+            case OR_FUNC_ELLIPTIC: { float t = 2.0f * tab[j]; v = t * zj; break; }
+            case OR_FUNC_RASTRIGIN: { float a = 2.0f * zj; float b = 62.8318531f * or_sin2pi(zj); v = a + b; break; }
+            case OR_FUNC_ACKLEY: { float a = ka * zj; float b = kb * or_sin2pi(zj); v = a + b; break; }
+            case OR_FUNC_ROSENBROCK:
+                if (j + 1 < D) {
+                    float zz = zj * zj;
+                    float a = zz - z[j + 1];
+                    float t = 400.0f * zj;
+                    t = t * a;
+                    float b = zj - 1.0f;
+                    b = 2.0f * b;
+                    v = t + b;
+                }
+                if (j > 0) {
+                    float zp = z[j - 1] * z[j - 1];
+                    float a = zp - zj;
+                    float t = 200.0f * a;
+                    v = v - t;
+                }
+                break;
+            case OR_FUNC_GRIEWANK: {
+                while (jn > j) {
+                    float t = z[jn] * tab[jn];
+                    float c = 1.0f - 2.0f * or_sinpi_sq(t);
+                    suf = suf * c;
+                    --jn;
+                }
+                float t = zj * tab[j];
+                float sn = or_sin2pi(t);
+                float a = zj / 2000.0f;
+                float b = pre[j] * suf;
+                b = b * sn;
+                float sc = 6.28318548f * tab[j];
+                b = b * sc;
+                v = a + b;
+                break;
+            }
+            case OR_FUNC_WEIERSTRASS: {
+                const float a = fabsf(zj);
+                float acc = 0.0f;
+                if (a < 8388608.0f) {
+                    float c = 6.28318548f;
+                    if (a < 2.32830644e-10f) {
+                        float p3 = 1.0f;
+                        for (int k = 0; k <= 20; ++k) {
+                            float r = a * p3;
+                            acc = fmaf(c, or_sin2pi(r), acc);
+                            c = c * 1.5f;
+                            p3 = p3 * 3.0f;
+                        }
+                    } else {
+                        uint32_t u;
+                        memcpy(&u, &a, 4);
+                        const int ex = (int)((u >> 23) & 0xFF) - 127;
+                        const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
+                        uint64_t y = m << (64 - (23 - ex));
+                        for (int k = 0; k <= 20; ++k) {
+                            float r = (float)(uint32_t)(y >> 32) * 2.3283064365386963e-10f;
+                            acc = fmaf(c, or_sin2pi(r), acc);
+                            c = c * 1.5f;
+                            y = y + (y << 1);
+                        }
+                    }
+                }
+                v = copysignf(1.0f, zj) * acc;
+                break;
+            }
+            case OR_FUNC_SCHWEFEL: {
+                float sq = sqrtf(fabsf(zj));
+                float t = sq * 0.159154937f;
+                float a = or_sin2pi(t);
+                float hs = 0.5f * sq;
+                float b = hs * or_sin2pi(t + 0.25f);
+                v = -(a + b);
+                break;
+            }
+            }
+            g[j] = v;
+        }
+    }
+    free(pre);
+}
+
+/* gx = Q^T dz: fmaf chain over j = 0 .. 2 HS - 1 (padded dimensions contribute +0) */
+void or_grad32(const or_task* t, int32_t n, const float* x, float* gx) {
+    const int D = t->dim;
+    float* w = (float*)malloc(sizeof(float) * (size_t)D * 4);
+    float* z = w + D;
+    float* tab = z + D;
+    float* dz = tab + D;
+    or_func_table(t->func, D, tab);
+    for (int i = 0; i < n; ++i) {
+        transform32(t, x + (size_t)i * D, w, z);
+        dfdz32(t->func, D, z, tab, dz);
+        float* g = gx + (size_t)i * D;
+        for (int d = 0; d < D; ++d) {
+            if (t->xform == OR_XFORM_ROTATE) {
+                float acc = 0.0f;
+                for (int j = 0; j < D; ++j) acc = fmaf(t->rot[(size_t)j * D + d], dz[j], acc);
+                g[d] = acc;
+            } else if (t->xform == OR_XFORM_DIAG) {
+                g[d] = fmaf(t->diag[d], dz[d], 0.0f);
+            } else {
+                g[d] = dz[d];
+            }
+        }
+    }
+    free(w);
+}

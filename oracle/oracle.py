@@ -30,7 +30,8 @@ class OrParams(ctypes.Structure):
                 ("n_islands", ctypes.c_int32), ("exchange_interval", ctypes.c_int32),
                 ("migrate_interval", ctypes.c_int32), ("migrate_count", ctypes.c_int32),
                 ("transfer_interval", ctypes.c_int32), ("transfer_count", ctypes.c_int32),
-                ("transfer_cr", ctypes.c_float), ("seed", ctypes.c_uint64)]
+                ("transfer_cr", ctypes.c_float), ("seed", ctypes.c_uint64), ("polish_interval", ctypes.c_int32),
+                ("polish_steps", ctypes.c_int32), ("polish_topk", ctypes.c_int32)]
 
 
 class CvrpInstance(ctypes.Structure):
@@ -71,6 +72,7 @@ def lib():
         L.or_eval32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
         L.or_eval64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
         L.or_grad64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
+        L.or_grad32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
         L.or_create.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, ctypes.POINTER(OrParams)]
         L.or_create.restype = ctypes.c_void_p
         L.or_destroy.argtypes = [ctypes.c_void_p]
@@ -163,6 +165,14 @@ def eval64(task, x):
     return f
 
 
+def grad32(task, x):
+    h = _TaskHolder(task)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, h.dim)
+    g = np.zeros_like(x)
+    lib().or_grad32(ctypes.byref(h.c), x.shape[0], _fp(x), _fp(g))
+    return g
+
+
 def grad64(task, x):
     h = _TaskHolder(task)
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, h.dim)
@@ -174,7 +184,7 @@ def grad64(task, x):
 def make_params(**kw):
     d = dict(algo=ALGO_DE_RAND_1_BIN, F=0.5, CR=0.9, eta_c=2.0, eta_m=5.0, pm=1.0, n_islands=1,
              exchange_interval=0, migrate_interval=0, migrate_count=0, transfer_interval=0,
-             transfer_count=0, transfer_cr=0.0, seed=0)
+             transfer_count=0, transfer_cr=0.0, seed=0, polish_interval=0, polish_steps=0, polish_topk=0)
     d.update(kw)
     return OrParams(**d)
 

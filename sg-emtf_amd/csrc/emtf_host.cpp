@@ -27,7 +27,11 @@
 EMTF_DECL_PAIR(2) EMTF_DECL_PAIR(4) EMTF_DECL_PAIR(7) EMTF_DECL_PAIR(8) EMTF_DECL_PAIR(12) EMTF_DECL_PAIR(13)
 EMTF_DECL_PAIR(16)
 extern "C" int emtf_lds_floats_kb(int pop, int kb);
-#define EMTF_DECL_OBJ(kb) extern "C" int emtf_launch_objective_kb##kb(const emtf::ObjArgs* a, int adjoint, hipStream_t s);
+#define EMTF_DECL_OBJ(kb)                                                                                   \
+    extern "C" int emtf_launch_objective_kb##kb(const emtf::ObjArgs* a, int adjoint, hipStream_t s);      \
+    extern "C" int emtf_polish_lds_floats_kb##kb(int pop, int waves);                                      \
+    extern "C" int emtf_launch_polish_kb##kb(const emtf::EpochArgs* args, const int* order, int n, int lds_bytes, \
+                                             hipStream_t stream);
 EMTF_DECL_OBJ(2) EMTF_DECL_OBJ(4) EMTF_DECL_OBJ(7) EMTF_DECL_OBJ(8) EMTF_DECL_OBJ(12) EMTF_DECL_OBJ(13) EMTF_DECL_OBJ(16)
 
 typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
@@ -44,6 +48,20 @@ static int pick_kb(int max_dim) {
 static launch_fn pick_launcher(int kb, bool ga) {
 #define EMTF_CASE(k) \
     case k: return ga ? emtf_launch_epoch_kb##k##_a2 : emtf_launch_epoch_kb##k##_a0;
+    switch (kb) {
+        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(7) EMTF_CASE(8) EMTF_CASE(12) EMTF_CASE(13) EMTF_CASE(16)
+    }
+#undef EMTF_CASE
+    return nullptr;
+}
+
+typedef int (*polish_lds_fn)(int, int);
+
+static launch_fn pick_polish(int kb, polish_lds_fn* lds) {
+#define EMTF_CASE(k)                          \
+    case k:                                   \
+        *lds = emtf_polish_lds_floats_kb##k; \
+        return emtf_launch_polish_kb##k;
     switch (kb) {
         EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(7) EMTF_CASE(8) EMTF_CASE(12) EMTF_CASE(13) EMTF_CASE(16)
     }
@@ -94,6 +112,8 @@ struct emtf_handle {
     float* d_scratch = nullptr;
     int* d_order = nullptr;      // task ids, most expensive first
     int lds_bytes = 0;
+    int polish_lds_bytes = 0;
+    int polish_waves = 0;        // 0: polish off
     int64_t mailbox_stride = 0;
     int kb = 2;
     int gen = 0;
@@ -125,6 +145,17 @@ static void func_table(int func, int dim, float* t) {
     }
 }
 
+static bool polish_at(const emtf_handle* h, int g) {
+    return h->polish_waves > 0 && g > 0 && g % h->p.polish_interval == 0;
+}
+
+static bool publish_at(const emtf_handle* h, int g) {
+    const int TI = h->p.transfer_interval;
+    return TI > 0 && h->T > 1 && h->p.transfer_count > 0 && g > 0 && g % TI == 0;
+}
+
+// Epoch g0 -> g1 (or the initial evaluation); the polish of generation g1 follows
+// in its own kernel, which then also publishes the elites.
 static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) {
     emtf::EpochArgs a{};
     a.tasks = h->d_tasks;
@@ -141,10 +172,20 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
     a.mailbox_stride = h->mailbox_stride;
     a.scratch = h->d_scratch;
     a.p = h->p;
+    const bool pol = !init && polish_at(h, g1);
+    a.publish = (!init && !pol && publish_at(h, g1)) ? 1 : 0;
     launch_fn fn = pick_launcher(h->kb, h->p.algo == emtf::A_GA_SBX_PM);
     if (!fn) return fail(EMTF_E_UNSUPPORTED, "no kernel built for this dimension");
-    const int rc = fn(&a, h->d_order, h->T, h->lds_bytes, stream);
+    int rc = fn(&a, h->d_order, h->T, h->lds_bytes, stream);
     if (rc != 0) return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
+    if (pol) {
+        polish_lds_fn plds = nullptr;
+        launch_fn pf = pick_polish(h->kb, &plds);
+        a.publish = publish_at(h, g1) ? 1 : 0;
+        a.polish_waves = h->polish_waves;
+        rc = pf(&a, h->d_order, h->T, h->polish_lds_bytes, stream);
+        if (rc != 0) return fail(EMTF_E_DEVICE, std::string("polish kernel launch: ") + hipGetErrorString((hipError_t)rc));
+    }
     return EMTF_OK;
 }
 
@@ -166,6 +207,8 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         prm->migrate_interval < 0 || prm->exchange_interval < 0)
         return fail(EMTF_E_INVALID, "negative interval or count");
     if (!(prm->F == prm->F) || !(prm->CR == prm->CR)) return fail(EMTF_E_INVALID, "F/CR is NaN");
+    if (prm->polish_interval < 0 || prm->polish_steps < 0 || prm->polish_topk < 0)
+        return fail(EMTF_E_INVALID, "negative polish knob");
 
     int max_dim = 0;
     for (int t = 0; t < n_tasks; ++t) {
@@ -211,9 +254,12 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     p.transfer_cr = prm->transfer_cr;
     p.seed_lo = (uint32_t)prm->seed;
     p.seed_hi = (uint32_t)(prm->seed >> 32);
+    p.polish_interval = prm->polish_interval;
+    p.polish_steps = prm->polish_steps;
+    p.polish_topk = prm->polish_topk;
 
     int64_t nX = 0, nF = 0, nQ = 0, nV = 0, nM = 0, nS = 0;
-    int lds = 0;
+    int lds = 0, max_pop = 0;
     h->tasks.resize(n_tasks);
     for (int t = 0; t < n_tasks; ++t) {
         const emtf_task_desc& d = tasks[t];
@@ -240,12 +286,30 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         nS += (int64_t)((d.pop + 31) & ~31) * RS;
         const int b = 4 * emtf_lds_floats_kb(d.pop, h->kb);
         if (b > lds) lds = b;
+        if (d.pop > max_pop) max_pop = d.pop;
     }
     if (lds > kMaxLdsBytes) {
         free_handle(h);
         return fail(EMTF_E_UNSUPPORTED, "population tile exceeds LDS (pop x dim too large)");
     }
     h->lds_bytes = lds;
+    if (prm->polish_interval > 0 && prm->polish_steps > 0 && prm->polish_topk > 0) {
+        // as many 32-elite waves as the elites need and the LDS holds
+        polish_lds_fn plds = nullptr;
+        if (!pick_polish(h->kb, &plds)) {
+            free_handle(h);
+            return fail(EMTF_E_UNSUPPORTED, "no polish kernel for this dimension");
+        }
+        const int k = std::min(prm->polish_topk, max_pop);
+        int w = std::min(4, (k + 31) / 32);
+        while (w > 1 && 4 * plds(max_pop, w) > kMaxLdsBytes) --w;
+        if (4 * plds(max_pop, w) > kMaxLdsBytes) {
+            free_handle(h);
+            return fail(EMTF_E_UNSUPPORTED, "polish staging exceeds LDS");
+        }
+        h->polish_waves = w;
+        h->polish_lds_bytes = 4 * plds(max_pop, w);
+    }
     std::vector<int> order(n_tasks);
     std::vector<double> cost(n_tasks);
     for (int t = 0; t < n_tasks; ++t) {
@@ -312,10 +376,15 @@ int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
     const int TI = h->p.transfer_interval;
     const bool xfer = TI > 0 && h->T > 1 && h->p.transfer_count > 0;
     const int g_end = h->gen + generations;
+    const int PI = h->polish_waves > 0 ? h->p.polish_interval : 0;
     while (h->gen < g_end) {
         int g1 = g_end;
         if (xfer) {
             const int nb = (h->gen / TI + 1) * TI;
+            if (nb < g1) g1 = nb;
+        }
+        if (PI > 0) {
+            const int nb = (h->gen / PI + 1) * PI;
             if (nb < g1) g1 = nb;
         }
         const int rc = launch(h, h->gen, g1, 0, s);

@@ -31,6 +31,7 @@ struct EngineParams {
     float transfer_cr;
     uint32_t seed_lo, seed_hi;
     uint32_t cr_threshold;  // floor(CR 2^16) clamped to [0, 65536]: keep the parent iff u16 >= it
+    int32_t polish_interval, polish_steps, polish_topk;  // gradient polish of elites (0 = off)
 };
 
 struct EpochArgs {
@@ -47,6 +48,8 @@ struct EpochArgs {
     int64_t mailbox_stride;  // elements per mailbox buffer
     float* scratch;
     EngineParams p;
+    int32_t publish;       // write the task's elites to the mailbox after generation g1
+    int32_t polish_waves;  // polish kernel: waves working on elites (1..4)
 };
 
 // Stand-alone objective / adjoint evaluation of n points (device pointers).

@@ -591,135 +591,6 @@ __device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& t
     __syncthreads();
 }
 
-template <int KB, int FUNC, int ALGO>
-__device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
-    using S = Shape<KB>;
-    constexpr int RS = S::RS, HS = S::HS, NT = S::NT;
-    const int tid = threadIdx.x;
-    Geo g;
-    g.D = td.dim;
-    g.Dh = (g.D + 1) >> 1;
-    g.pop = td.pop;
-    g.popP = (g.pop + 31) & ~31;
-    g.ntiles = g.popP / 32;
-    g.m = g.pop / (a.p.n_islands > 0 ? a.p.n_islands : 1);
-    g.modD.init((uint32_t)g.D);
-    g.modM.init((uint32_t)g.m);
-    Lds L;
-    L.X = smem;
-    L.Q = L.X + g.popP * RS;
-    L.o = L.Q + 32 * NT * RS;
-    L.tab = L.o + RS;
-    L.fit = L.tab + 32 * NT;
-    L.ftmp = L.fit + g.popP;
-    L.ord = reinterpret_cast<int*>(L.ftmp + g.popP);
-    L.aux = L.ord + g.popP;
-    const Rng rng{a.p.seed_lo, a.p.seed_hi, (uint32_t)task};
-    const int lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
-
-    const int nz = L.fit - smem;  // X, Q, o, tab
-    for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
-    __syncthreads();
-    const float* Qg = a.Q + td.q_off;
-    for (int e = tid; e < g.D * g.D; e += kThreads) {
-        const int j = e / g.D, d = e - j * g.D;
-        L.Q[j * RS + (d & 1) * HS + (d >> 1)] = Qg[e];
-    }
-    for (int d = tid; d < g.D; d += kThreads) {
-        L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
-        L.tab[d] = a.tab[td.v_off + d];
-    }
-    if (a.init) {
-        const float lo = td.lo, rng_ = td.hi - td.lo;
-        for (int e = tid; e < g.pop * g.D; e += kThreads) {
-            const int i = e / g.D, d = e - i * g.D;
-            const float u = uniform01(rng.word(0u, P_INIT, (uint32_t)i, 0u, (uint32_t)d));
-            const float t1 = u * rng_;
-            L.X[i * RS + (d & 1) * HS + (d >> 1)] = t1 + lo;
-        }
-        __syncthreads();
-        for (int it = wave; it < g.ntiles; it += kThreads / 64) {
-            const int i = it * 32 + col;
-            float x[HS];
-            load_half_row<KB>(x, L.X + i * RS + h * HS);
-            const float f = evaluate_tile<KB, FUNC>(x, g, L, col, h);
-            if (i < g.pop && h == 0) L.fit[i] = f;
-        }
-    } else {
-        const float* Xg = a.X + td.x_off;
-        for (int e = tid; e < g.pop * g.D; e += kThreads) {
-            const int i = e / g.D, d = e - i * g.D;
-            L.X[i * RS + (d & 1) * HS + (d >> 1)] = Xg[e];
-        }
-        for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = a.fit[td.fit_off + i];
-    }
-    __syncthreads();
-
-    const int TI = a.p.transfer_interval;
-    const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
-    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) transfer_in<KB, FUNC>(a, td, g, L, rng, tid);
-
-    const bool islands = a.p.n_islands > 1;
-    for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
-        if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
-            island_exchange<KB>(a, td, g, L, rng, gen, tid);
-        generation_step<KB, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
-        if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
-            island_migration<KB>(a, g, L, tid);
-    }
-
-    float* Xg = a.X + td.x_off;
-    for (int e = tid; e < g.pop * g.D; e += kThreads) {
-        const int i = e / g.D, d = e - i * g.D;
-        Xg[e] = L.X[i * RS + (d & 1) * HS + (d >> 1)];
-    }
-    for (int i = tid; i < g.pop; i += kThreads) a.fit[td.fit_off + i] = L.fit[i];
-    if (xfer_on && a.g1 > 0 && a.g1 % TI == 0) {
-        __syncthreads();
-        rank_islands(L, g.pop, g.pop, tid);
-        __syncthreads();
-        const int k = a.p.transfer_count;
-        float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
-        for (int e = tid; e < k * g.D; e += kThreads) {
-            const int r = e / g.D, d = e - r * g.D;
-            mb[e] = L.X[L.ord[r] * RS + (d & 1) * HS + (d >> 1)];
-        }
-    }
-}
-
-// One workgroup per task; `order` lists the tasks longest-first so the
-// expensive objectives are dispatched before the cheap ones fill the gaps.
-// Register budget: with KB <= 8 (dims <= 64) three 4-wave workgroups fit in LDS per
-// CU, so the allocator is asked for 3 waves per SIMD (<= 168 VGPR+AGPR); larger
-// rows are LDS-bound to fewer workgroups and keep the default budget.
-#if !defined(EMTF_WAVES_PER_EU) && defined(EMTF_KB) && EMTF_KB <= 8
-#define EMTF_WAVES_PER_EU 3
-#endif
-#ifdef EMTF_WAVES_PER_EU
-#define EMTF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EMTF_WAVES_PER_EU, EMTF_WAVES_PER_EU)))
-#else
-#define EMTF_OCCUPANCY
-#endif
-
-template <int KB, int ALGO>
-__global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
-    extern __shared__ float smem[];
-    const int task = order[blockIdx.x];
-    const TaskDev td = a.tasks[task];
-    switch (td.func) {
-    case F_SPHERE: epoch_body<KB, F_SPHERE, ALGO>(a, smem, task, td); break;
-    case F_ROSENBROCK: epoch_body<KB, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
-    case F_ACKLEY: epoch_body<KB, F_ACKLEY, ALGO>(a, smem, task, td); break;
-    case F_RASTRIGIN: epoch_body<KB, F_RASTRIGIN, ALGO>(a, smem, task, td); break;
-    case F_GRIEWANK: epoch_body<KB, F_GRIEWANK, ALGO>(a, smem, task, td); break;
-    case F_WEIERSTRASS: epoch_body<KB, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
-    case F_SCHWEFEL: epoch_body<KB, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
-    case F_ELLIPTIC: epoch_body<KB, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
-    default: break;
-    }
-}
-
-
 // ---------------------------------------------------------------------------
 // Stand-alone objective and adjoint (emtf_objective_fwd / emtf_objective_adjoint).
 // A wave evaluates 32 points.  The forward pass is the engine's evaluator (bit
@@ -826,6 +697,342 @@ __device__ __forceinline__ void dfdz(float (&g)[4 * KB], const float* z, float* 
         g[s] = v;
     }
 }
+
+
+// Orders the LDS traffic of one wave around a cross-lane exchange.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Gradient of the lane's point xrow (its parity half at xrow[h HS ..]): z = Q (x - o)
+// on MFMA staged in zrow, dz = df/dz (pre: Griewank prefix products), gx = Q^T dz
+// on MFMA staged back into zrow in dimension order; returns |gx|^2 summed
+// sequentially over the dimensions.
+template <int KB, int FUNC>
+__device__ __forceinline__ float gradient_tile(const float* xrow, const Geo& g, const Lds& L, float* zrow, float* pre,
+                                               int col, int h) {
+    using S = Shape<KB>;
+    constexpr int RS = S::RS, HS = S::HS, NT = S::NT;
+    {
+        f16v acc[NT];
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < HS; ++s) {
+            const float w = xrow[h * HS + s] - L.o[h * HS + s];
+#pragma unroll
+            for (int jt = 0; jt < NT; ++jt)
+                acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(L.Q[(32 * jt + col) * RS + h * HS + s], w, acc[jt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) zrow[32 * jt + 8 * (r >> 2) + 4 * h + (r & 3)] = acc[jt][r];
+    }
+    wave_sync();
+    float dz[HS];
+    dfdz<KB, FUNC>(dz, zrow, pre, g.D, L.tab, h);
+    f16v gacc[NT];
+#pragma unroll
+    for (int dt = 0; dt < NT; ++dt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) gacc[dt][r] = 0.0f;
+#pragma unroll
+    for (int s = 0; s < HS; ++s) {
+        const int j = 2 * s + h;
+#pragma unroll
+        for (int dt = 0; dt < NT; ++dt) {
+            const int d = 32 * dt + col;
+            const float qa = (j < 32 * NT) ? L.Q[j * RS + (d & 1) * HS + (d >> 1)] : 0.0f;
+            gacc[dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa, dz[s], gacc[dt], 0, 0, 0);
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int dt = 0; dt < NT; ++dt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) zrow[32 * dt + 8 * (r >> 2) + 4 * h + (r & 3)] = gacc[dt][r];
+    wave_sync();
+    float g2 = 0.0f;
+    for (int d = 0; d < g.D; ++d) g2 = fmaf(zrow[d], zrow[d], g2);
+    return g2;
+}
+
+// Armijo polish of the top-k elites of one task, run between epochs after
+// generation g1 (restates the oracle's polish_task): from a Polyak step
+// alpha = f / |g|^2 along -g, halve alpha until f(clamp(x - alpha g)) <=
+// f - 1e-4 alpha |g|^2 (at most 24 halvings), take the step, repeat.  One lane
+// pair per elite, `polish_waves` waves of 32 elites; the loops are uniform per
+// wave so each MFMA runs with the full wave, and finished lanes keep their point.
+// At a transfer boundary the task's elites are published afterwards.
+template <int KB, int FUNC>
+__device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, const TaskDev& td) {
+    using S = Shape<KB>;
+    constexpr int RS = S::RS, HS = S::HS, NT = S::NT, ZS = ObjShape<KB>::ZS;
+    const int tid = threadIdx.x, lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
+    Geo g{};
+    g.D = td.dim;
+    g.pop = td.pop;
+    g.popP = (g.pop + 31) & ~31;
+    Lds L{};
+    L.Q = smem;
+    L.o = L.Q + 32 * NT * RS;
+    L.tab = L.o + RS;
+    L.fit = L.tab + 32 * NT;
+    L.ord = reinterpret_cast<int*>(L.fit + g.popP);
+    float* stage = reinterpret_cast<float*>(L.ord + g.popP);
+    const int nz = 32 * NT * RS + RS + 32 * NT;
+    for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
+    __syncthreads();
+    const float* Qg = a.Q + td.q_off;
+    for (int e = tid; e < g.D * g.D; e += kThreads) {
+        const int j = e / g.D, d = e - j * g.D;
+        L.Q[j * RS + (d & 1) * HS + (d >> 1)] = Qg[e];
+    }
+    for (int d = tid; d < g.D; d += kThreads) {
+        L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
+        L.tab[d] = a.tab[td.v_off + d];
+    }
+    float* fitg = a.fit + td.fit_off;
+    float* Xg = a.X + td.x_off;
+    for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = fitg[i];
+    __syncthreads();
+    rank_islands(L, g.pop, g.pop, tid);
+    __syncthreads();
+    const int K = a.p.polish_topk < g.pop ? a.p.polish_topk : g.pop;
+    const int ntc = (K + 31) / 32, nw = a.polish_waves;
+    if (wave < nw) {
+        float* xrow = stage + (96 * wave + col) * ZS;  // current point, parity halves as in L.X
+        float* zrow = stage + (96 * wave + 32 + col) * ZS;
+        float* pre = stage + (96 * wave + 64 + col) * ZS;
+        for (int ct = wave; ct < ntc; ct += nw) {
+            const int e = ct * 32 + col;
+            const bool valid = e < K;
+            const int i = L.ord[valid ? e : 0];
+            float* xr = Xg + (int64_t)i * g.D;
+#pragma unroll
+            for (int s = 0; s < HS; ++s) xrow[h * HS + s] = (2 * s + h < g.D) ? xr[2 * s + h] : 0.0f;
+            float f = L.fit[i];
+            bool active = valid;
+            wave_sync();
+            for (int step = 0; step < a.p.polish_steps; ++step) {
+                const float g2 = gradient_tile<KB, FUNC>(xrow, g, L, zrow, pre, col, h);
+                if (!(g2 > 0.0f) || !(f > 0.0f)) active = false;
+                float alpha = f / g2;
+                bool done = false;
+                float fnew = f;
+                float xn[HS];
+                for (int b = 0; b < 24; ++b) {
+                    asm volatile("" ::: "memory");  // re-read the staged rows: keeps them out of registers
+#pragma unroll
+                    for (int s = 0; s < HS; ++s) {
+                        const float st = alpha * zrow[2 * s + h];
+                        xn[s] = (2 * s + h < g.D) ? clampf(xrow[h * HS + s] - st, td.lo, td.hi) : 0.0f;
+                    }
+                    const float fn = evaluate_tile<KB, FUNC>(xn, g, L, col, h);
+                    const float c1 = 1.0e-4f * alpha;
+                    const float c2 = c1 * g2;
+                    const float thr = f - c2;
+                    if (active && !done) {
+                        if (fn <= thr) {
+                            done = true;
+                            fnew = fn;
+                        } else {
+                            alpha = alpha * 0.5f;
+                        }
+                    }
+                    if (!__any(active && !done)) break;
+                }
+                // the accepted point is the last one formed: alpha stops halving at acceptance
+#pragma unroll
+                for (int s = 0; s < HS; ++s) {
+                    const float st = alpha * zrow[2 * s + h];
+                    xn[s] = (2 * s + h < g.D) ? clampf(xrow[h * HS + s] - st, td.lo, td.hi) : 0.0f;
+                }
+                wave_sync();
+                if (done) {
+#pragma unroll
+                    for (int s = 0; s < HS; ++s) xrow[h * HS + s] = xn[s];
+                    f = fnew;
+                } else {
+                    active = false;
+                }
+                wave_sync();
+            }
+            if (valid) {
+#pragma unroll
+                for (int s = 0; s < HS; ++s)
+                    if (2 * s + h < g.D) xr[2 * s + h] = xrow[h * HS + s];
+                if (h == 0) {
+                    L.fit[i] = f;
+                    fitg[i] = f;
+                }
+            }
+            wave_sync();
+        }
+    }
+    __syncthreads();
+    if (a.publish) {
+        rank_islands(L, g.pop, g.pop, tid);
+        __syncthreads();
+        const int k = a.p.transfer_count, TI = a.p.transfer_interval;
+        float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
+        for (int e = tid; e < k * g.D; e += kThreads) {
+            const int r = e / g.D, d = e - r * g.D;
+            mb[e] = Xg[(int64_t)L.ord[r] * g.D + d];
+        }
+    }
+}
+
+template <int KB>
+__global__ __launch_bounds__(kThreads) void polish_kernel(EpochArgs a, const int* __restrict__ order) {
+    extern __shared__ float smem[];
+    const TaskDev td = a.tasks[order[blockIdx.x]];
+    switch (td.func) {
+    case F_SPHERE: polish_body<KB, F_SPHERE>(a, smem, td); break;
+    case F_ROSENBROCK: polish_body<KB, F_ROSENBROCK>(a, smem, td); break;
+    case F_ACKLEY: polish_body<KB, F_ACKLEY>(a, smem, td); break;
+    case F_RASTRIGIN: polish_body<KB, F_RASTRIGIN>(a, smem, td); break;
+    case F_GRIEWANK: polish_body<KB, F_GRIEWANK>(a, smem, td); break;
+    case F_WEIERSTRASS: polish_body<KB, F_WEIERSTRASS>(a, smem, td); break;
+    case F_SCHWEFEL: polish_body<KB, F_SCHWEFEL>(a, smem, td); break;
+    case F_ELLIPTIC: polish_body<KB, F_ELLIPTIC>(a, smem, td); break;
+    default: break;
+    }
+}
+
+template <int KB, int FUNC, int ALGO>
+__device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
+    using S = Shape<KB>;
+    constexpr int RS = S::RS, HS = S::HS, NT = S::NT;
+    const int tid = threadIdx.x;
+    Geo g;
+    g.D = td.dim;
+    g.Dh = (g.D + 1) >> 1;
+    g.pop = td.pop;
+    g.popP = (g.pop + 31) & ~31;
+    g.ntiles = g.popP / 32;
+    g.m = g.pop / (a.p.n_islands > 0 ? a.p.n_islands : 1);
+    g.modD.init((uint32_t)g.D);
+    g.modM.init((uint32_t)g.m);
+    Lds L;
+    L.X = smem;
+    L.Q = L.X + g.popP * RS;
+    L.o = L.Q + 32 * NT * RS;
+    L.tab = L.o + RS;
+    L.fit = L.tab + 32 * NT;
+    L.ftmp = L.fit + g.popP;
+    L.ord = reinterpret_cast<int*>(L.ftmp + g.popP);
+    L.aux = L.ord + g.popP;
+    const Rng rng{a.p.seed_lo, a.p.seed_hi, (uint32_t)task};
+    const int lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
+
+    const int nz = L.fit - smem;  // X, Q, o, tab
+    for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
+    __syncthreads();
+    const float* Qg = a.Q + td.q_off;
+    for (int e = tid; e < g.D * g.D; e += kThreads) {
+        const int j = e / g.D, d = e - j * g.D;
+        L.Q[j * RS + (d & 1) * HS + (d >> 1)] = Qg[e];
+    }
+    for (int d = tid; d < g.D; d += kThreads) {
+        L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
+        L.tab[d] = a.tab[td.v_off + d];
+    }
+    if (a.init) {
+        const float lo = td.lo, rng_ = td.hi - td.lo;
+        for (int e = tid; e < g.pop * g.D; e += kThreads) {
+            const int i = e / g.D, d = e - i * g.D;
+            const float u = uniform01(rng.word(0u, P_INIT, (uint32_t)i, 0u, (uint32_t)d));
+            const float t1 = u * rng_;
+            L.X[i * RS + (d & 1) * HS + (d >> 1)] = t1 + lo;
+        }
+        __syncthreads();
+        for (int it = wave; it < g.ntiles; it += kThreads / 64) {
+            const int i = it * 32 + col;
+            float x[HS];
+            load_half_row<KB>(x, L.X + i * RS + h * HS);
+            const float f = evaluate_tile<KB, FUNC>(x, g, L, col, h);
+            if (i < g.pop && h == 0) L.fit[i] = f;
+        }
+    } else {
+        const float* Xg = a.X + td.x_off;
+        for (int e = tid; e < g.pop * g.D; e += kThreads) {
+            const int i = e / g.D, d = e - i * g.D;
+            L.X[i * RS + (d & 1) * HS + (d >> 1)] = Xg[e];
+        }
+        for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = a.fit[td.fit_off + i];
+    }
+    __syncthreads();
+
+    const int TI = a.p.transfer_interval;
+    const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
+    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) transfer_in<KB, FUNC>(a, td, g, L, rng, tid);
+
+    const bool islands = a.p.n_islands > 1;
+    for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
+        if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
+            island_exchange<KB>(a, td, g, L, rng, gen, tid);
+        generation_step<KB, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
+        if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
+            island_migration<KB>(a, g, L, tid);
+    }
+
+    float* Xg = a.X + td.x_off;
+    for (int e = tid; e < g.pop * g.D; e += kThreads) {
+        const int i = e / g.D, d = e - i * g.D;
+        Xg[e] = L.X[i * RS + (d & 1) * HS + (d >> 1)];
+    }
+    for (int i = tid; i < g.pop; i += kThreads) a.fit[td.fit_off + i] = L.fit[i];
+    if (a.publish) {
+        __syncthreads();
+        rank_islands(L, g.pop, g.pop, tid);
+        __syncthreads();
+        const int k = a.p.transfer_count;
+        float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
+        for (int e = tid; e < k * g.D; e += kThreads) {
+            const int r = e / g.D, d = e - r * g.D;
+            mb[e] = L.X[L.ord[r] * RS + (d & 1) * HS + (d >> 1)];
+        }
+    }
+}
+
+// One workgroup per task; `order` lists the tasks longest-first so the
+// expensive objectives are dispatched before the cheap ones fill the gaps.
+// Register budget: with KB <= 8 (dims <= 64) three 4-wave workgroups fit in LDS per
+// CU, so the allocator is asked for 3 waves per SIMD (<= 168 VGPR+AGPR); larger
+// rows are LDS-bound to fewer workgroups and keep the default budget.
+#if !defined(EMTF_WAVES_PER_EU) && defined(EMTF_KB) && EMTF_KB <= 8
+#define EMTF_WAVES_PER_EU 3
+#endif
+#ifdef EMTF_WAVES_PER_EU
+#define EMTF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EMTF_WAVES_PER_EU, EMTF_WAVES_PER_EU)))
+#else
+#define EMTF_OCCUPANCY
+#endif
+
+template <int KB, int ALGO>
+__global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
+    extern __shared__ float smem[];
+    const int task = order[blockIdx.x];
+    const TaskDev td = a.tasks[task];
+    switch (td.func) {
+    case F_SPHERE: epoch_body<KB, F_SPHERE, ALGO>(a, smem, task, td); break;
+    case F_ROSENBROCK: epoch_body<KB, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
+    case F_ACKLEY: epoch_body<KB, F_ACKLEY, ALGO>(a, smem, task, td); break;
+    case F_RASTRIGIN: epoch_body<KB, F_RASTRIGIN, ALGO>(a, smem, task, td); break;
+    case F_GRIEWANK: epoch_body<KB, F_GRIEWANK, ALGO>(a, smem, task, td); break;
+    case F_WEIERSTRASS: epoch_body<KB, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
+    case F_SCHWEFEL: epoch_body<KB, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
+    case F_ELLIPTIC: epoch_body<KB, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
+    default: break;
+    }
+}
+
 
 template <int KB, int FUNC, bool ADJ>
 __global__ __launch_bounds__(2 * ObjShape<KB>::PB) void objective_kernel(ObjArgs a) {
@@ -988,6 +1195,28 @@ static hipError_t launch_obj_any(const emtf::ObjArgs& a, hipStream_t s) {
 
 #define EMTF_OBJ_NAME2(a, b) a##b
 #define EMTF_OBJ_NAME(a, b) EMTF_OBJ_NAME2(a, b)
+
+// Polish kernel LDS: Q, o, tab, fit, ord, then three staged rows (point, z /
+// gradient, Griewank prefix products) per elite of each of the `waves` active waves.
+extern "C" int EMTF_OBJ_NAME(emtf_polish_lds_floats_kb, EMTF_KB)(int pop, int waves) {
+    using S = emtf::Shape<EMTF_KB>;
+    const int popP = (pop + 31) & ~31;
+    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * popP + waves * 96 * emtf::ObjShape<EMTF_KB>::ZS;
+}
+
+extern "C" int EMTF_OBJ_NAME(emtf_launch_polish_kb, EMTF_KB)(const emtf::EpochArgs* args, const int* order, int n,
+                                                             int lds_bytes, hipStream_t stream) {
+    static bool lds_ok = false;
+    auto k = &emtf::polish_kernel<EMTF_KB>;
+    if (!lds_ok) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return (int)e;
+        lds_ok = true;
+    }
+    hipLaunchKernelGGL(k, dim3(n), dim3(emtf::kThreads), lds_bytes, stream, *args, order);
+    return (int)hipGetLastError();
+}
 extern "C" int EMTF_OBJ_NAME(emtf_launch_objective_kb, EMTF_KB)(const emtf::ObjArgs* a, int adjoint, hipStream_t s) {
     return (int)(adjoint ? launch_obj_any<true>(*a, s) : launch_obj_any<false>(*a, s));
 }

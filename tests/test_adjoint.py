@@ -101,6 +101,20 @@ def test_weierstrass_grad64_vs_mpmath(oracle_mod):
             assert abs(float(num) - g[i, d]) <= 2e-5 * (1 + abs(float(num)))
 
 
+@pytest.mark.parametrize("func", range(8))
+def test_grad32_vs_grad64(oracle_mod, func):
+    """The oracle's binary32 gradient (the device adjoint's restatement) tracks FP64."""
+    for rotate in (False, True):
+        exact_z = func == 5 and rotate
+        t = _task(func, 30, rotate and not exact_z, 400 + func)
+        x = _points(t, 16, func, 1e-3 if func == 5 else (0.5 if func in (3, 4) else 5.0))
+        g32 = oracle_mod.grad32(t, x).astype(np.float64)
+        g64 = oracle_mod.grad64(t, x.astype(np.float64))
+        err = np.linalg.norm(g32 - g64, axis=1)
+        scale = np.linalg.norm(g64, axis=1) + 1e-3
+        assert np.all(err <= 1e-4 * scale), (func, rotate, float((err / scale).max()))
+
+
 # ----------------------------------------------------------------------- GPU
 def _cuda(a):
     return torch.tensor(a, dtype=torch.float32, device="cuda:0") if a is not None else None
@@ -118,6 +132,23 @@ def test_gpu_forward_bit_exact(oracle_mod, D):
             f = forward_raw(func, _cuda(x), _cuda(t.get("rot")), _cuda(t["shift"])).cpu().numpy()
             ref = oracle_mod.eval32(t, x)
             assert np.array_equal(f.view(np.uint32), ref.view(np.uint32)), (func, rotate)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D", [7, 50, 100, 128])
+def test_gpu_adjoint_bit_exact(oracle_mod, D):
+    """emtf_objective_adjoint with unit upstream gradient equals the oracle's grad32 bit for bit."""
+    from emtf.objective import adjoint_raw
+    for func in range(8):
+        for rotate in (False, True):
+            t = _task(func, D, rotate, 500 + func)
+            x = _points(t, 70, func, 1e-3 if func == 5 else 3.0)
+            print("case: adjoint bits D", D, "func", func, "rotate", rotate, flush=True)
+            gf = torch.ones(70, dtype=torch.float32, device="cuda:0")
+            g = adjoint_raw(func, _cuda(x), gf, _cuda(t.get("rot")), _cuda(t["shift"])).cpu().numpy()
+            ref = oracle_mod.grad32(t, x)
+            assert np.array_equal(g.view(np.uint32), ref.view(np.uint32)), \
+                (func, rotate, int(np.sum(g.view(np.uint32) != ref.view(np.uint32))))
 
 
 @pytest.mark.gpu
