@@ -37,8 +37,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-gens", type=int, default=100, help="generations of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
-    ap.add_argument("--workload", default="plain", choices=["plain", "cvrp"],
-                    help="plain: the rotated continuous workload (default); cvrp: the CVRP GA workload")
+    ap.add_argument("--workload", default="plain", choices=["plain", "cvrp", "polish"],
+                    help="plain: the rotated continuous workload (default); cvrp: the CVRP GA workload; "
+                         "polish: time to mean error 1e-6, adjoint polish vs plain DE at D=100")
+    ap.add_argument("--seeds", type=int, default=10, help="polish workload: seeds 0..N-1")
+    ap.add_argument("--gens", type=int, default=3000, help="polish workload: generations per run")
     ap.add_argument("--only-func", default=None, help="give every task this objective (per-objective timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters)")
@@ -118,6 +121,107 @@ def bench_cvrp(a):
     print(json.dumps(out))
 
 
+def _converge(tasks, params, gens, chunk, target):
+    """Run in chunks of `chunk` generations; GPU time (HIP events) and FP64 error after each."""
+    import numpy as np
+    import emtf
+    from emtf import hip
+    from emtf.convergence import error64
+    eng = emtf.Engine(tasks, **params)
+    hip.device_synchronize()
+    t_ms, errs = [0.0], [error64(tasks, eng.best()[1])]
+    for _ in range(gens // chunk):
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record()
+        eng.run(chunk)
+        e1.record()
+        hip.device_synchronize()
+        t_ms.append(t_ms[-1] + e0.elapsed_ms(e1))
+        errs.append(error64(tasks, eng.best()[1]))
+    eng.close()
+    errs = np.array(errs)  # [points, tasks]
+    return np.array(t_ms), errs
+
+
+def _time_to(t_ms, mean_err, target):
+    import numpy as np
+    hit = np.nonzero(mean_err <= target)[0]
+    return float(t_ms[hit[0]]) if len(hit) else None
+
+
+def bench_polish(a):
+    """Adjoint-polish workload: 1024 rotated tasks, D=100, pop 128, DE/rand/1/bin with 4 elites
+    transferred every 20 generations; the polished variant adds 5 Armijo steps on the top 4
+    elites of every task every 50 generations.  Error = float64 re-evaluation of each task's
+    best point (Schwefel excluded: no known minimum inside the box).  Reports the GPU time to
+    mean error <= 1e-6 of both variants over seeds 0..N-1 and the per-objective picture."""
+    import numpy as np
+    import emtf
+    from emtf import tasks as tg
+    target, chunk = 1e-6, 50
+    names = ["sphere", "rosenbrock", "ackley", "rastrigin", "griewank", "weierstrass", "schwefel", "elliptic"]
+    base = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=20, transfer_count=4)
+    res = {"plain": [], "polish": []}
+    per_func = {m: {n: [] for n in names} for m in res}
+    final = {m: [] for m in res}
+    for seed in range(a.seeds):
+        tasks = tg.benchmark_tasks(1024, 100, 128, rotate=True, seed=seed)
+        funcs = np.array([t["func"] for t in tasks])
+        for mode in res:
+            p = dict(base, seed=seed)
+            if mode == "polish":
+                p.update(polish_interval=50, polish_steps=5, polish_topk=4)
+            t_ms, errs = _converge(tasks, p, a.gens, chunk, target)
+            live = funcs != 6
+            mean_err = np.nanmean(errs[:, live], axis=1)
+            res[mode].append(_time_to(t_ms, mean_err, target))
+            final[mode].append((float(mean_err[-1]), float(t_ms[-1])))
+            for f, n in enumerate(names):
+                if f == 6:
+                    continue
+                m = np.mean(errs[:, funcs == f], axis=1)
+                per_func[mode][n].append((_time_to(t_ms, m, target), float(np.mean(errs[-1, funcs == f] <= target)),
+                                          float(m[-1])))
+            print(f"seed {seed} {mode}: mean error {mean_err[-1]:.3e} after {a.gens} gens, "
+                  f"{t_ms[-1]:.0f} ms GPU, time to 1e-6: {res[mode][-1]}", file=sys.stderr, flush=True)
+
+    def summary(mode):
+        tt = [v for v in res[mode] if v is not None]
+        return {"runs_reaching": len(tt), "mean_ms_to_target": float(np.mean(tt)) if tt else None,
+                "final_mean_error": float(np.mean([e for e, _ in final[mode]])),
+                "gpu_ms_full_run": float(np.mean([t for _, t in final[mode]]))}
+    funcs_out = {}
+    for n in names:
+        if n == "schwefel":
+            continue
+        row = {}
+        for mode in res:
+            tt = [t for t, _, _ in per_func[mode][n] if t is not None]
+            row[mode] = {"runs_reaching": len(tt), "mean_ms_to_target": float(np.mean(tt)) if tt else None,
+                         "frac_tasks_at_target": float(np.mean([fr for _, fr, _ in per_func[mode][n]])),
+                         "final_mean_error": float(np.mean([e for _, _, e in per_func[mode][n]]))}
+        funcs_out[n] = row
+    # objectives whose mean error reaches the target in every run of both variants
+    both = [n for n, r in funcs_out.items()
+            if r["plain"]["runs_reaching"] == a.seeds and r["polish"]["runs_reaching"] == a.seeds]
+    subset_ratio = (float(np.exp(np.mean([np.log(funcs_out[n]["polish"]["mean_ms_to_target"] /
+                                                 funcs_out[n]["plain"]["mean_ms_to_target"]) for n in both])))
+                    if both else None)
+    sp, sq = summary("plain"), summary("polish")
+    ratio = (sq["mean_ms_to_target"] / sp["mean_ms_to_target"]
+             if sp["mean_ms_to_target"] and sq["mean_ms_to_target"] else None)
+    out = {"metric": "GPU time to mean error 1e-6: adjoint polish / plain DE", "value": ratio, "unit": "ratio",
+           "n_gpus": 1, "steps": a.gens // chunk, "warmup": 0, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "adjoint polish: 1024 rotated tasks, D=100, pop 128, DE/rand/1/bin F=0.5 CR=0.9, "
+                                  "4 elites every 20 gens; polish 5 Armijo steps on the top 4 elites every 50 gens",
+                      "generations": a.gens, "seeds": a.seeds, "target": target,
+                      "error": "float64 f(best) - f*, mean over non-Schwefel tasks"},
+           "plain": sp, "polish": sq, "per_objective": funcs_out,
+           "ratio_on_objectives_reached_by_both": {"objectives": both, "geomean_time_ratio": subset_ratio}}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -125,6 +229,9 @@ def main():
         return
     if a.workload == "cvrp":
         bench_cvrp(a)
+        return
+    if a.workload == "polish":
+        bench_polish(a)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine shards tasks across "

@@ -29,7 +29,7 @@ EMTF_DECL_PAIR(16)
 extern "C" int emtf_lds_floats_kb(int pop, int kb);
 #define EMTF_DECL_OBJ(kb)                                                                                   \
     extern "C" int emtf_launch_objective_kb##kb(const emtf::ObjArgs* a, int adjoint, hipStream_t s);      \
-    extern "C" int emtf_polish_lds_floats_kb##kb(int pop, int waves);                                      \
+    extern "C" int emtf_polish_lds_floats_kb##kb(int pop, int waves, int cpe);                             \
     extern "C" int emtf_launch_polish_kb##kb(const emtf::EpochArgs* args, const int* order, int n, int lds_bytes, \
                                              hipStream_t stream);
 EMTF_DECL_OBJ(2) EMTF_DECL_OBJ(4) EMTF_DECL_OBJ(7) EMTF_DECL_OBJ(8) EMTF_DECL_OBJ(12) EMTF_DECL_OBJ(13) EMTF_DECL_OBJ(16)
@@ -55,7 +55,7 @@ static launch_fn pick_launcher(int kb, bool ga) {
     return nullptr;
 }
 
-typedef int (*polish_lds_fn)(int, int);
+typedef int (*polish_lds_fn)(int, int, int);
 
 static launch_fn pick_polish(int kb, polish_lds_fn* lds) {
 #define EMTF_CASE(k)                          \
@@ -114,6 +114,7 @@ struct emtf_handle {
     int lds_bytes = 0;
     int polish_lds_bytes = 0;
     int polish_waves = 0;        // 0: polish off
+    int polish_cpe = 1;
     int64_t mailbox_stride = 0;
     int kb = 2;
     int gen = 0;
@@ -183,6 +184,7 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
         launch_fn pf = pick_polish(h->kb, &plds);
         a.publish = publish_at(h, g1) ? 1 : 0;
         a.polish_waves = h->polish_waves;
+        a.polish_cpe = h->polish_cpe;
         rc = pf(&a, h->d_order, h->T, h->polish_lds_bytes, stream);
         if (rc != 0) return fail(EMTF_E_DEVICE, std::string("polish kernel launch: ") + hipGetErrorString((hipError_t)rc));
     }
@@ -300,15 +302,22 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
             free_handle(h);
             return fail(EMTF_E_UNSUPPORTED, "no polish kernel for this dimension");
         }
+        // spread the elites over the four waves and let the spare MFMA columns try
+        // further step sizes: cpe columns per elite, the largest power of two <= 32
+        // with k elites fitting 4 waves of 32 / cpe
         const int k = std::min(prm->polish_topk, max_pop);
-        int w = std::min(4, (k + 31) / 32);
-        while (w > 1 && 4 * plds(max_pop, w) > kMaxLdsBytes) --w;
-        if (4 * plds(max_pop, w) > kMaxLdsBytes) {
+        int cpe = 32;
+        while (cpe > 1 && cpe * k > 128) cpe /= 2;
+        const int epw = 32 / cpe;
+        int w = std::min(4, (k + epw - 1) / epw);
+        while (w > 1 && 4 * plds(max_pop, w, cpe) > kMaxLdsBytes) --w;
+        if (4 * plds(max_pop, w, cpe) > kMaxLdsBytes) {
             free_handle(h);
             return fail(EMTF_E_UNSUPPORTED, "polish staging exceeds LDS");
         }
         h->polish_waves = w;
-        h->polish_lds_bytes = 4 * plds(max_pop, w);
+        h->polish_cpe = cpe;
+        h->polish_lds_bytes = 4 * plds(max_pop, w, cpe);
     }
     std::vector<int> order(n_tasks);
     std::vector<double> cost(n_tasks);

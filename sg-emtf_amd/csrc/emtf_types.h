@@ -50,6 +50,7 @@ struct EpochArgs {
     EngineParams p;
     int32_t publish;       // write the task's elites to the mailbox after generation g1
     int32_t polish_waves;  // polish kernel: waves working on elites (1..4)
+    int32_t polish_cpe;    // polish kernel: MFMA columns per elite (power of two, 1..32)
 };
 
 // Stand-alone objective / adjoint evaluation of n points (device pointers).

@@ -804,13 +804,18 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
     rank_islands(L, g.pop, g.pop, tid);
     __syncthreads();
     const int K = a.p.polish_topk < g.pop ? a.p.polish_topk : g.pop;
-    const int ntc = (K + 31) / 32, nw = a.polish_waves;
+    // cpe columns per elite (a power of two, <= 32): the line search tries cpe step
+    // sizes per round, and the first accepted one in halving order is taken, which
+    // is the step the one-at-a-time search would take.
+    const int cpe = a.polish_cpe, epw = 32 / cpe;  // elites per wave pass
+    const int ntc = (K + epw - 1) / epw, nw = a.polish_waves;
     if (wave < nw) {
-        float* xrow = stage + (96 * wave + col) * ZS;  // current point, parity halves as in L.X
-        float* zrow = stage + (96 * wave + 32 + col) * ZS;
-        float* pre = stage + (96 * wave + 64 + col) * ZS;
+        const int slot = col / cpe, sub = col - slot * cpe;
+        float* xrow = stage + (3 * epw * wave + slot) * ZS;  // current point, parity halves as in L.X
+        float* zrow = stage + (3 * epw * wave + epw + slot) * ZS;
+        float* pre = stage + (3 * epw * wave + 2 * epw + slot) * ZS;
         for (int ct = wave; ct < ntc; ct += nw) {
-            const int e = ct * 32 + col;
+            const int e = ct * epw + slot;
             const bool valid = e < K;
             const int i = L.ord[valid ? e : 0];
             float* xr = Xg + (int64_t)i * g.D;
@@ -822,12 +827,15 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
             for (int step = 0; step < a.p.polish_steps; ++step) {
                 const float g2 = gradient_tile<KB, FUNC>(xrow, g, L, zrow, pre, col, h);
                 if (!(g2 > 0.0f) || !(f > 0.0f)) active = false;
-                float alpha = f / g2;
+                const float alpha0 = f / g2;
                 bool done = false;
-                float fnew = f;
+                float fnew = f, alpha_ok = alpha0;
                 float xn[HS];
-                for (int b = 0; b < 24; ++b) {
+                for (int b0 = 0; b0 < 24; b0 += cpe) {
                     asm volatile("" ::: "memory");  // re-read the staged rows: keeps them out of registers
+                    const int bb = b0 + sub;
+                    float alpha = alpha0;
+                    for (int q = 0; q < bb; ++q) alpha = alpha * 0.5f;
 #pragma unroll
                     for (int s = 0; s < HS; ++s) {
                         const float st = alpha * zrow[2 * s + h];
@@ -837,20 +845,24 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
                     const float c1 = 1.0e-4f * alpha;
                     const float c2 = c1 * g2;
                     const float thr = f - c2;
-                    if (active && !done) {
-                        if (fn <= thr) {
-                            done = true;
-                            fnew = fn;
-                        } else {
-                            alpha = alpha * 0.5f;
-                        }
+                    const bool ok = active && !done && bb < 24 && fn <= thr;
+                    // first accepting column of this elite's group (lane half 0 carries the vote)
+                    const uint32_t vote = (uint32_t)__ballot(ok);
+                    const uint32_t grp = (vote >> (slot * cpe)) & ((cpe == 32) ? 0xFFFFFFFFu : ((1u << cpe) - 1u));
+                    const int win = grp ? __builtin_ctz(grp) : 0;
+                    const int src = slot * cpe + win;  // both lane halves follow half 0
+                    const float fn_w = __shfl(fn, src);
+                    const float al_w = __shfl(alpha, src);
+                    if (active && !done && grp) {
+                        done = true;
+                        fnew = fn_w;
+                        alpha_ok = al_w;
                     }
                     if (!__any(active && !done)) break;
                 }
-                // the accepted point is the last one formed: alpha stops halving at acceptance
 #pragma unroll
                 for (int s = 0; s < HS; ++s) {
-                    const float st = alpha * zrow[2 * s + h];
+                    const float st = alpha_ok * zrow[2 * s + h];
                     xn[s] = (2 * s + h < g.D) ? clampf(xrow[h * HS + s] - st, td.lo, td.hi) : 0.0f;
                 }
                 wave_sync();
@@ -863,7 +875,7 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
                 }
                 wave_sync();
             }
-            if (valid) {
+            if (valid && sub == 0) {
 #pragma unroll
                 for (int s = 0; s < HS; ++s)
                     if (2 * s + h < g.D) xr[2 * s + h] = xrow[h * HS + s];
@@ -1197,11 +1209,12 @@ static hipError_t launch_obj_any(const emtf::ObjArgs& a, hipStream_t s) {
 #define EMTF_OBJ_NAME(a, b) EMTF_OBJ_NAME2(a, b)
 
 // Polish kernel LDS: Q, o, tab, fit, ord, then three staged rows (point, z /
-// gradient, Griewank prefix products) per elite of each of the `waves` active waves.
-extern "C" int EMTF_OBJ_NAME(emtf_polish_lds_floats_kb, EMTF_KB)(int pop, int waves) {
+// gradient, Griewank prefix products) per elite slot (32 / cpe) of each active wave.
+extern "C" int EMTF_OBJ_NAME(emtf_polish_lds_floats_kb, EMTF_KB)(int pop, int waves, int cpe) {
     using S = emtf::Shape<EMTF_KB>;
     const int popP = (pop + 31) & ~31;
-    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * popP + waves * 96 * emtf::ObjShape<EMTF_KB>::ZS;
+    const int rows = 3 * (32 / cpe) * waves;
+    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * popP + rows * emtf::ObjShape<EMTF_KB>::ZS;
 }
 
 extern "C" int EMTF_OBJ_NAME(emtf_launch_polish_kb, EMTF_KB)(const emtf::EpochArgs* args, const int* order, int n,

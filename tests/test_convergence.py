@@ -1,0 +1,83 @@
+"""Convergence records, ANO and the FP64 error metric.
+
+ANO is pinned by the reference's own data: its T = 100 record file (18 runs x
+100 tasks) and the curve its evaluation program printed for it must come out
+digit for digit.  error64 is pinned against the oracle's independent FP64
+evaluator and against the known minima.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from emtf import convergence as cv
+from emtf import tasks as tg
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_ano_reproduces_reference_curve():
+    R = np.load(os.path.join(GOLD, "ano_T100_records.npz"))["records"]
+    assert R.shape == (18, 100, 22)
+    with open(os.path.join(GOLD, "ano_T100_expected.txt")) as fh:
+        expected = fh.read().strip()
+    assert cv.format_ano(cv.ano(R)).strip() == expected
+
+
+def test_ano_constant_task_contributes_zero_and_range():
+    R = np.ones((2, 3, 5), np.float32)
+    R[:, 1, :] = np.array([10, 5, 2, 1, 20], np.float32)
+    curve = cv.ano(R)
+    # task 1: (v - 1) / (20 - 1); tasks 0 and 2 are constant
+    ref = np.array([9, 4, 1, 0], np.float32) / np.float32(19) / np.float32(3)
+    assert np.allclose(curve, ref, rtol=1e-6)
+    assert curve[-1] == 0
+
+
+def test_record_roundtrip(tmp_path):
+    R = np.load(os.path.join(GOLD, "ano_T100_records.npz"))["records"][:3]
+    path = str(tmp_path / "rec.txt")
+    for run in R:
+        cv.write_record(path, run)
+    back = cv.read_record(path)
+    assert np.array_equal(back.view(np.uint32), R.view(np.uint32))
+    lines = open(path).read().split("\n")
+    assert lines[0].endswith(" ") and lines[100] == ""
+
+
+def test_error64_matches_oracle_eval64(oracle_mod):
+    tasks = tg.benchmark_tasks(16, 30, 32, rotate=True, seed=5)
+    rng = np.random.default_rng(1)
+    for t in tasks:
+        x = (t["shift"][None, :] + rng.normal(0, 2.0, (8, 30))).astype(np.float32)
+        a = cv.objective64(t, x)
+        b = oracle_mod.eval64(t, x.astype(np.float64))
+        assert np.allclose(a, b, rtol=1e-9, atol=1e-9), (t["func"], a, b)
+
+
+def test_error64_at_minimum():
+    tasks = tg.benchmark_tasks(8, 20, 32, rotate=True, seed=9)
+    xb = np.zeros((8, 24))
+    for i, t in enumerate(tasks):
+        z = np.ones(20) if t["func"] == 1 else np.zeros(20)
+        xb[i, :20] = t["shift"].astype(np.float64) + np.asarray(t["rot"], np.float64).T @ z
+    err = cv.error64(tasks, xb)
+    assert np.isnan(err[5])  # Schwefel: no known minimum in the box
+    assert np.all(np.abs(np.delete(err, 5)) < 1e-9), err
+
+
+@pytest.mark.gpu
+def test_recorder_on_engine(tmp_path):
+    import emtf
+    tasks = tg.benchmark_tasks(16, 20, 64, rotate=True, seed=3)
+    eng = emtf.Engine(tasks, seed=2, transfer_interval=10, transfer_count=2)
+    rec = cv.ConvergenceRecorder(eng, 10)
+    rec.run(100)
+    rows = rec.rows()
+    assert rows.shape == (16, 12)
+    assert np.all(np.diff(rows[:, :-1], axis=1) <= 0)  # elitist: the best never gets worse
+    assert np.all(rows[:, -1] >= rows[:, 0])
+    path = str(tmp_path / "rec.txt")
+    cv.write_record(path, rows)
+    curve = cv.ano(cv.read_record(path))
+    assert curve[0] == pytest.approx(np.mean((rows[:, 0] - rows[:, -2]) / (rows[:, -1] - rows[:, -2])), rel=1e-4)

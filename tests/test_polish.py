@@ -73,7 +73,9 @@ def _compare(gpu, orc, tag):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dim,pop,topk,steps", [(50, 64, 4, 3), (20, 128, 40, 2), (128, 128, 64, 2), (7, 32, 32, 5)])
+@pytest.mark.parametrize("dim,pop,topk,steps", [(50, 64, 4, 3), (20, 128, 40, 2), (128, 128, 64, 2), (7, 32, 32, 5),
+                                                 (50, 64, 3, 4), (33, 96, 12, 3), (100, 128, 8, 5), (50, 128, 1, 5),
+                                                 (64, 256, 200, 1)])
 def test_gpu_polish_bit_exact(oracle_mod, dim, pop, topk, steps):
     tasks = tg.benchmark_tasks(8, dim, pop, rotate=True, seed=30 + dim)
     params = dict(seed=12, F=0.5, CR=0.9, transfer_interval=10, transfer_count=2, polish_interval=5,
