@@ -1,8 +1,10 @@
 """Builds libemtf.so (HIP kernels + C ABI) in-tree for gfx950.
 
-island.hip is compiled once per (gene-block count, algorithm family) pair; each
-object holds one kernel that dispatches per task to an objective-specialised
-body.  The objects are linked with the host runtime into sg-emtf_amd/libemtf.so.
+island.hip is compiled once per (gene-block count, algorithm family) pair of the
+LDS-resident kernel and once per (row-tile count, algorithm family) pair of the
+wide kernel; each object holds one kernel that dispatches per task to an
+objective-specialised body.  The objects are linked with the host runtime into
+sg-emtf_amd/libemtf.so.
 """
 import os
 import subprocess
@@ -21,6 +23,8 @@ COMMON = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off", "-f
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 # (gene blocks per parity half KB: dims <= 8 KB; algorithm family: 0 = DE, 2 = GA)
 VARIANTS = [(kb, al) for kb in (2, 4, 7, 8, 12, 13, 16) for al in (0, 2)]
+# wide kernel: (32-row tiles NT: dims <= 32 NT, rows in global memory; algorithm family)
+WIDE = [(nt, al) for nt in (2, 4, 6, 8) for al in (0, 2)]
 
 
 def _run(cmd):
@@ -37,6 +41,9 @@ def build(verbose=False, jobs=None, out=None, extra=()):
     units = [([HIPCC] + COMMON + list(extra) + [f"-DEMTF_KB={kb}", f"-DEMTF_ALGO={al}", "-c",
                                   "-o", os.path.join(OBJ, f"island_kb{kb}_a{al}.o"), os.path.join(CSRC, "island.hip")])
              for kb, al in VARIANTS]
+    units += [([HIPCC] + COMMON + list(extra) + [f"-DEMTF_WIDE_NT={nt}", f"-DEMTF_ALGO={al}", "-c",
+                                   "-o", os.path.join(OBJ, f"island_w{nt}_a{al}.o"), os.path.join(CSRC, "island.hip")])
+              for nt, al in WIDE]
     for name in ("emtf_host.cpp", "cvrp_host.cpp", "cvrp.hip"):
         units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, name.split(".")[0] + ".o"), os.path.join(CSRC, name)])
     with ThreadPoolExecutor(jobs) as ex:
@@ -48,6 +55,7 @@ def build(verbose=False, jobs=None, out=None, extra=()):
         if verbose and msg:
             print(msg)
     objs = [os.path.join(OBJ, f"island_kb{kb}_a{al}.o") for kb, al in VARIANTS] + \
+        [os.path.join(OBJ, f"island_w{nt}_a{al}.o") for nt, al in WIDE] + \
         [os.path.join(OBJ, n + ".o") for n in ("emtf_host", "cvrp_host", "cvrp")]
     rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
     if rc != 0:

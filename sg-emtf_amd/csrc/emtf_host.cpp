@@ -2,10 +2,15 @@
 //
 // Owns the device state of a batch of tasks (one contiguous arena per array,
 // per-task offsets in a descriptor table) and schedules epochs of the island
-// kernel.  An epoch never crosses a transfer boundary, so the kernel boundary
-// is the synchronisation point of the elite mailbox: the launch that ends at
-// generation g writes mailbox buffer (g / interval) & 1, the next launch reads
-// it while writing the other one.
+// kernels.  Tasks are grouped into buckets by the kernel that runs them: the
+// LDS-resident kernel built for the smallest row geometry KB >= ceil(dim / 8)
+// whose population fits LDS, or else the wide kernel (rows in global memory)
+// for the smallest 32-row tile count NT >= dim / 32.  Each epoch launches every
+// bucket on its own stream, forked from and joined back into the caller's
+// stream.  An epoch never crosses a transfer boundary, so the epoch boundary is
+// the synchronisation point of the elite mailbox: the launches that end at
+// generation g write mailbox buffer (g / interval) & 1, the next ones read it
+// while writing the other one.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -27,6 +32,12 @@
 EMTF_DECL_PAIR(2) EMTF_DECL_PAIR(4) EMTF_DECL_PAIR(7) EMTF_DECL_PAIR(8) EMTF_DECL_PAIR(12) EMTF_DECL_PAIR(13)
 EMTF_DECL_PAIR(16)
 extern "C" int emtf_lds_floats_kb(int pop, int kb);
+#define EMTF_DECL_WIDE(nt, al)                                                                             \
+    extern "C" int emtf_launch_wide_nt##nt##_a##al(const emtf::EpochArgs* args, const int* order, int n,   \
+                                                   int lds_bytes, hipStream_t stream);                     \
+    extern "C" int emtf_wide_lds_floats_nt##nt##_a##al(int pop);
+#define EMTF_DECL_WPAIR(nt) EMTF_DECL_WIDE(nt, 0) EMTF_DECL_WIDE(nt, 2)
+EMTF_DECL_WPAIR(2) EMTF_DECL_WPAIR(4) EMTF_DECL_WPAIR(6) EMTF_DECL_WPAIR(8)
 #define EMTF_DECL_OBJ(kb)                                                                                   \
     extern "C" int emtf_launch_objective_kb##kb(const emtf::ObjArgs* a, int adjoint, hipStream_t s);      \
     extern "C" int emtf_polish_lds_floats_kb##kb(int pop, int waves, int cpe);                             \
@@ -38,11 +49,31 @@ typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream
 
 static const int kKB[] = {2, 4, 7, 8, 12, 13, 16};
 
+static const int kNT[] = {2, 4, 6, 8};
+
 static int pick_kb(int max_dim) {
     const int need = (max_dim + 7) / 8;
     for (int kb : kKB)
         if (kb >= need) return kb;
     return -1;
+}
+
+static int pick_nt(int dim) {
+    for (int nt : kNT)
+        if (32 * nt >= dim) return nt;
+    return -1;
+}
+
+static launch_fn pick_wide(int nt, bool ga, int (**lds)(int)) {
+#define EMTF_CASE(k)                                                               \
+    case k:                                                                        \
+        *lds = ga ? emtf_wide_lds_floats_nt##k##_a2 : emtf_wide_lds_floats_nt##k##_a0; \
+        return ga ? emtf_launch_wide_nt##k##_a2 : emtf_launch_wide_nt##k##_a0;
+    switch (nt) {
+        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(6) EMTF_CASE(8)
+    }
+#undef EMTF_CASE
+    return nullptr;
 }
 
 static launch_fn pick_launcher(int kb, bool ga) {
@@ -98,6 +129,20 @@ constexpr int kMaxLdsBytes = 160 * 1024;
 
 }  // namespace
 
+// Tasks run by one kernel variant: d_order[offset .. offset + count), most expensive first.
+struct Bucket {
+    bool wide = false;
+    int geo = 0;                 // KB (LDS path) or NT (wide path)
+    launch_fn fn = nullptr;
+    int lds_bytes = 0;
+    int offset = 0, count = 0;
+    double cost = 0.0;
+    int polish_lds_bytes = 0;
+    int polish_waves = 0;        // 0: polish off
+    int polish_cpe = 1;
+    int stream = 0;              // index into the handle's streams (0 = the caller's)
+};
+
 struct emtf_handle {
     int T = 0;
     emtf::EngineParams p{};
@@ -110,16 +155,17 @@ struct emtf_handle {
     float* d_tab = nullptr;
     float* d_mailbox = nullptr;
     float* d_scratch = nullptr;
-    int* d_order = nullptr;      // task ids, most expensive first
-    int lds_bytes = 0;
-    int polish_lds_bytes = 0;
-    int polish_waves = 0;        // 0: polish off
-    int polish_cpe = 1;
+    float* d_qw = nullptr;       // padded Q images of the wide tasks
+    int* d_order = nullptr;      // task ids grouped by bucket
+    std::vector<Bucket> buckets;
+    bool polish = false;
     int64_t mailbox_stride = 0;
-    int kb = 2;
     int gen = 0;
     int max_dim = 0;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> side;  // extra streams for buckets 1.. (non-blocking, default priority)
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
 };
 
 static void free_handle(emtf_handle* h) {
@@ -132,7 +178,11 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_tab);
     (void)hipFree(h->d_mailbox);
     (void)hipFree(h->d_scratch);
+    (void)hipFree(h->d_qw);
     (void)hipFree(h->d_order);
+    for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     delete h;
 }
 
@@ -147,7 +197,7 @@ static void func_table(int func, int dim, float* t) {
 }
 
 static bool polish_at(const emtf_handle* h, int g) {
-    return h->polish_waves > 0 && g > 0 && g % h->p.polish_interval == 0;
+    return h->polish && g > 0 && g % h->p.polish_interval == 0;
 }
 
 static bool publish_at(const emtf_handle* h, int g) {
@@ -155,8 +205,9 @@ static bool publish_at(const emtf_handle* h, int g) {
     return TI > 0 && h->T > 1 && h->p.transfer_count > 0 && g > 0 && g % TI == 0;
 }
 
-// Epoch g0 -> g1 (or the initial evaluation); the polish of generation g1 follows
-// in its own kernel, which then also publishes the elites.
+// Epoch g0 -> g1 (or the initial evaluation) of every bucket, each on its stream;
+// the polish of generation g1 follows in its own kernel, which then also
+// publishes the elites.
 static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) {
     emtf::EpochArgs a{};
     a.tasks = h->d_tasks;
@@ -172,21 +223,36 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
     a.mailbox = h->d_mailbox;
     a.mailbox_stride = h->mailbox_stride;
     a.scratch = h->d_scratch;
+    a.Qw = h->d_qw;
     a.p = h->p;
     const bool pol = !init && polish_at(h, g1);
-    a.publish = (!init && !pol && publish_at(h, g1)) ? 1 : 0;
-    launch_fn fn = pick_launcher(h->kb, h->p.algo == emtf::A_GA_SBX_PM);
-    if (!fn) return fail(EMTF_E_UNSUPPORTED, "no kernel built for this dimension");
-    int rc = fn(&a, h->d_order, h->T, h->lds_bytes, stream);
-    if (rc != 0) return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
-    if (pol) {
-        polish_lds_fn plds = nullptr;
-        launch_fn pf = pick_polish(h->kb, &plds);
-        a.publish = publish_at(h, g1) ? 1 : 0;
-        a.polish_waves = h->polish_waves;
-        a.polish_cpe = h->polish_cpe;
-        rc = pf(&a, h->d_order, h->T, h->polish_lds_bytes, stream);
-        if (rc != 0) return fail(EMTF_E_DEVICE, std::string("polish kernel launch: ") + hipGetErrorString((hipError_t)rc));
+    const bool pub = !init && publish_at(h, g1);
+    const bool fork = !h->side.empty();
+    if (fork) {
+        HIP_OR_FAIL(hipEventRecord(h->ev_fork, stream));
+        for (hipStream_t s : h->side) HIP_OR_FAIL(hipStreamWaitEvent(s, h->ev_fork, 0));
+    }
+    for (const Bucket& b : h->buckets) {
+        hipStream_t s = b.stream == 0 ? stream : h->side[b.stream - 1];
+        a.publish = (pub && !(pol && b.polish_waves > 0)) ? 1 : 0;
+        int rc = b.fn(&a, h->d_order + b.offset, b.count, b.lds_bytes, s);
+        if (rc != 0) return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
+        if (pol && b.polish_waves > 0) {
+            int (*plds)(int, int, int) = nullptr;
+            launch_fn pf = pick_polish(b.geo, &plds);
+            emtf::EpochArgs ap = a;
+            ap.publish = pub ? 1 : 0;
+            ap.polish_waves = b.polish_waves;
+            ap.polish_cpe = b.polish_cpe;
+            rc = pf(&ap, h->d_order + b.offset, b.count, b.polish_lds_bytes, s);
+            if (rc != 0) return fail(EMTF_E_DEVICE, std::string("polish kernel launch: ") + hipGetErrorString((hipError_t)rc));
+        }
+    }
+    if (fork) {
+        for (size_t k = 0; k < h->side.size(); ++k) {
+            HIP_OR_FAIL(hipEventRecord(h->ev_join[k], h->side[k]));
+            HIP_OR_FAIL(hipStreamWaitEvent(stream, h->ev_join[k], 0));
+        }
     }
     return EMTF_OK;
 }
@@ -218,7 +284,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         char buf[160];
         if (d.func < 0 || d.func > EMTF_ELLIPTIC) { snprintf(buf, sizeof buf, "task %d: unknown func %d", t, d.func); return fail(EMTF_E_INVALID, buf); }
         if (d.dim < 1) { snprintf(buf, sizeof buf, "task %d: dim %d < 1", t, d.dim); return fail(EMTF_E_INVALID, buf); }
-        if (d.dim > 128) { snprintf(buf, sizeof buf, "task %d: dim %d > 128 not built", t, d.dim); return fail(EMTF_E_UNSUPPORTED, buf); }
+        if (d.dim > 256) { snprintf(buf, sizeof buf, "task %d: dim %d > 256 not built", t, d.dim); return fail(EMTF_E_UNSUPPORTED, buf); }
         if (d.pop < 2 || d.pop % nis != 0) { snprintf(buf, sizeof buf, "task %d: pop %d not a multiple of n_islands %d", t, d.pop, nis); return fail(EMTF_E_INVALID, buf); }
         if (d.pop > 1024) { snprintf(buf, sizeof buf, "task %d: pop %d > 1024", t, d.pop); return fail(EMTF_E_UNSUPPORTED, buf); }
         if (!(d.lo < d.hi) || !std::isfinite(d.lo) || !std::isfinite(d.hi)) { snprintf(buf, sizeof buf, "task %d: bad box", t); return fail(EMTF_E_INVALID, buf); }
@@ -235,7 +301,6 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     emtf_handle* h = new emtf_handle();
     h->T = n_tasks;
     h->max_dim = max_dim;
-    h->kb = pick_kb(max_dim);
     emtf::EngineParams& p = h->p;
     p.algo = prm->algo;
     p.F = prm->F;
@@ -260,11 +325,65 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     p.polish_steps = prm->polish_steps;
     p.polish_topk = prm->polish_topk;
 
-    int64_t nX = 0, nF = 0, nQ = 0, nV = 0, nM = 0, nS = 0;
-    int lds = 0, max_pop = 0;
+    // bucket of every task: LDS kernel of the smallest KB whose population fits, else the wide kernel
+    const bool ga = prm->algo == EMTF_GA_SBX_PM;
+    const bool want_polish = prm->polish_interval > 0 && prm->polish_steps > 0 && prm->polish_topk > 0;
+    std::vector<int> bucket_of(n_tasks), wide_nt(n_tasks, -1);
+    std::vector<double> cost(n_tasks);
+    for (int t = 0; t < n_tasks; ++t) {
+        const emtf_task_desc& d = tasks[t];
+        const int kb = pick_kb(d.dim);
+        bool wide = kb < 0 || 4 * emtf_lds_floats_kb(d.pop, kb) > kMaxLdsBytes;
+        const int geo = wide ? pick_nt(d.dim) : kb;
+        if (wide && want_polish) {
+            free_handle(h);
+            char buf[160];
+            snprintf(buf, sizeof buf, "task %d: gradient polish needs the population in LDS (dim %d, pop %d)", t, d.dim, d.pop);
+            return fail(EMTF_E_UNSUPPORTED, buf);
+        }
+        int bi = -1;
+        for (size_t k = 0; k < h->buckets.size(); ++k)
+            if (h->buckets[k].wide == wide && h->buckets[k].geo == geo) bi = (int)k;
+        if (bi < 0) {
+            Bucket b;
+            b.wide = wide;
+            b.geo = geo;
+            if (wide) {
+                int (*wl)(int) = nullptr;
+                b.fn = pick_wide(geo, ga, &wl);
+                b.lds_bytes = 0;
+            } else {
+                b.fn = pick_launcher(geo, ga);
+            }
+            if (!b.fn) {
+                free_handle(h);
+                return fail(EMTF_E_UNSUPPORTED, "no kernel built for this dimension");
+            }
+            h->buckets.push_back(b);
+            bi = (int)h->buckets.size() - 1;
+        }
+        bucket_of[t] = bi;
+        if (wide) wide_nt[t] = geo;
+        Bucket& b = h->buckets[bi];
+        if (wide) {
+            int (*wl)(int) = nullptr;
+            pick_wide(geo, ga, &wl);
+            b.lds_bytes = std::max(b.lds_bytes, 4 * wl(d.pop));
+        } else {
+            b.lds_bytes = std::max(b.lds_bytes, 4 * emtf_lds_floats_kb(d.pop, geo));
+        }
+        // relative cost per generation: objective work per gene + the rotation GEMM
+        const int dk = wide ? 32 * geo : 8 * geo;
+        cost[t] = func_cost(d.func) * d.pop * (double)d.dim + 2.0 * d.pop * (double)dk * dk / 64.0;
+        b.cost += cost[t];
+        ++b.count;
+    }
+
+    int64_t nX = 0, nF = 0, nQ = 0, nV = 0, nM = 0, nS = 0, nW = 0;
     h->tasks.resize(n_tasks);
     for (int t = 0; t < n_tasks; ++t) {
         const emtf_task_desc& d = tasks[t];
+        const Bucket& b = h->buckets[bucket_of[t]];
         emtf::TaskDev& td = h->tasks[t];
         std::memset(&td, 0, sizeof td);
         td.func = d.func;
@@ -279,54 +398,90 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         td.v_off = nV;
         td.mb_off = nM;
         td.scr_off = nS;
-        const int RS = 8 * h->kb + 4;
+        td.qw_off = nW;
+        const int popP = (d.pop + 31) & ~31;
         nX += (int64_t)d.pop * d.dim;
         nF += d.pop;
         nQ += (int64_t)d.dim * d.dim;
         nV += d.dim;
         nM += (int64_t)p.transfer_count * d.dim;
-        nS += (int64_t)((d.pop + 31) & ~31) * RS;
-        const int b = 4 * emtf_lds_floats_kb(d.pop, h->kb);
-        if (b > lds) lds = b;
-        if (d.pop > max_pop) max_pop = d.pop;
+        if (b.wide) {
+            const int RS = 32 * b.geo + 4;
+            nS += 2 * (int64_t)popP * RS;  // population rows, trial rows
+            nW += (int64_t)32 * b.geo * RS;
+        } else {
+            nS += (int64_t)popP * (8 * b.geo + 4);
+        }
     }
-    if (lds > kMaxLdsBytes) {
+    for (size_t k = 0; k < h->buckets.size(); ++k)
+        if (h->buckets[k].lds_bytes > kMaxLdsBytes) {
+            free_handle(h);
+            return fail(EMTF_E_UNSUPPORTED, "population tile exceeds LDS (pop x dim too large)");
+        }
+    if (want_polish) {
+        h->polish = true;
+        for (int t = 0; t < n_tasks; ++t) {
+            Bucket& b = h->buckets[bucket_of[t]];
+            if (b.polish_waves > 0) continue;
+            polish_lds_fn plds = nullptr;
+            if (!pick_polish(b.geo, &plds)) {
+                free_handle(h);
+                return fail(EMTF_E_UNSUPPORTED, "no polish kernel for this dimension");
+            }
+            int max_pop = 0;
+            for (int u = 0; u < n_tasks; ++u)
+                if (bucket_of[u] == bucket_of[t]) max_pop = std::max(max_pop, tasks[u].pop);
+            // spread the elites over the four waves and let the spare MFMA columns try
+            // further step sizes: cpe columns per elite, the largest power of two <= 32
+            // with k elites fitting 4 waves of 32 / cpe
+            const int k = std::min(prm->polish_topk, max_pop);
+            int cpe = 32;
+            while (cpe > 1 && cpe * k > 128) cpe /= 2;
+            const int epw = 32 / cpe;
+            int w = std::min(4, (k + epw - 1) / epw);
+            while (w > 1 && 4 * plds(max_pop, w, cpe) > kMaxLdsBytes) --w;
+            if (4 * plds(max_pop, w, cpe) > kMaxLdsBytes) {
+                free_handle(h);
+                return fail(EMTF_E_UNSUPPORTED, "polish staging exceeds LDS");
+            }
+            b.polish_waves = w;
+            b.polish_cpe = cpe;
+            b.polish_lds_bytes = 4 * plds(max_pop, w, cpe);
+        }
+    }
+    // buckets most expensive first, tasks inside a bucket most expensive first
+    std::vector<int> border(h->buckets.size());
+    for (size_t k = 0; k < border.size(); ++k) border[k] = (int)k;
+    std::stable_sort(border.begin(), border.end(), [&](int x, int y) { return h->buckets[x].cost > h->buckets[y].cost; });
+    std::vector<int> order;
+    order.reserve(n_tasks);
+    std::vector<Bucket> sorted;
+    for (size_t r = 0; r < border.size(); ++r) {
+        Bucket b = h->buckets[border[r]];
+        b.offset = (int)order.size();
+        b.stream = (int)(r % 4);  // the caller's stream and up to three more
+        for (int t = 0; t < n_tasks; ++t)
+            if (bucket_of[t] == border[r]) order.push_back(t);
+        std::stable_sort(order.begin() + b.offset, order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+        sorted.push_back(b);
+    }
+    h->buckets = sorted;
+    const int n_side = std::min<int>(3, (int)h->buckets.size() - 1);
+    for (int k = 0; k < n_side; ++k) {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            free_handle(h);
+            return fail(EMTF_E_DEVICE, "stream / event creation failed");
+        }
+        h->side.push_back(s);
+        h->ev_join.push_back(ev);
+    }
+    if (n_side > 0 && hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess) {
         free_handle(h);
-        return fail(EMTF_E_UNSUPPORTED, "population tile exceeds LDS (pop x dim too large)");
+        return fail(EMTF_E_DEVICE, "event creation failed");
     }
-    h->lds_bytes = lds;
-    if (prm->polish_interval > 0 && prm->polish_steps > 0 && prm->polish_topk > 0) {
-        // as many 32-elite waves as the elites need and the LDS holds
-        polish_lds_fn plds = nullptr;
-        if (!pick_polish(h->kb, &plds)) {
-            free_handle(h);
-            return fail(EMTF_E_UNSUPPORTED, "no polish kernel for this dimension");
-        }
-        // spread the elites over the four waves and let the spare MFMA columns try
-        // further step sizes: cpe columns per elite, the largest power of two <= 32
-        // with k elites fitting 4 waves of 32 / cpe
-        const int k = std::min(prm->polish_topk, max_pop);
-        int cpe = 32;
-        while (cpe > 1 && cpe * k > 128) cpe /= 2;
-        const int epw = 32 / cpe;
-        int w = std::min(4, (k + epw - 1) / epw);
-        while (w > 1 && 4 * plds(max_pop, w, cpe) > kMaxLdsBytes) --w;
-        if (4 * plds(max_pop, w, cpe) > kMaxLdsBytes) {
-            free_handle(h);
-            return fail(EMTF_E_UNSUPPORTED, "polish staging exceeds LDS");
-        }
-        h->polish_waves = w;
-        h->polish_cpe = cpe;
-        h->polish_lds_bytes = 4 * plds(max_pop, w, cpe);
-    }
-    std::vector<int> order(n_tasks);
-    std::vector<double> cost(n_tasks);
-    for (int t = 0; t < n_tasks; ++t) {
-        order[t] = t;
-        cost[t] = func_cost(tasks[t].func) * tasks[t].pop * (double)tasks[t].dim +
-                  2.0 * tasks[t].pop * (double)tasks[t].dim * tasks[t].dim / 64.0;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
     h->mailbox_stride = nM > 0 ? nM : 1;
 
     std::vector<float> Q((size_t)nQ, 0.0f), V((size_t)nV, 0.0f), TB((size_t)nV, 0.0f);
@@ -341,6 +496,18 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         if (d.shift) std::memcpy(V.data() + td.v_off, d.shift, sizeof(float) * d.dim);
         func_table(d.func, d.dim, TB.data() + td.v_off);
     }
+    // wide tasks: Q rows in the kernels' [even | odd] row layout, zero padded to 32 NT rows
+    std::vector<float> QW((size_t)nW, 0.0f);
+    for (int t = 0; t < n_tasks; ++t) {
+        const emtf::TaskDev& td = h->tasks[t];
+        const int nt = wide_nt[t];
+        if (nt < 0) continue;
+        const int RS = 32 * nt + 4, HS = 16 * nt;
+        const float* q = Q.data() + td.q_off;
+        float* w = QW.data() + td.qw_off;
+        for (int j = 0; j < td.dim; ++j)
+            for (int d2 = 0; d2 < td.dim; ++d2) w[(size_t)j * RS + (d2 & 1) * HS + (d2 >> 1)] = q[(size_t)j * td.dim + d2];
+    }
 
     auto alloc = [&](float** ptr, int64_t n) -> int {
         const hipError_t e = hipMalloc((void**)ptr, (size_t)(n > 0 ? n : 1) * sizeof(float));
@@ -350,7 +517,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     int rc = EMTF_OK;
     if ((rc = alloc(&h->d_X, nX)) || (rc = alloc(&h->d_fit, nF)) || (rc = alloc(&h->d_Q, nQ)) ||
         (rc = alloc(&h->d_shift, nV)) || (rc = alloc(&h->d_tab, nV)) || (rc = alloc(&h->d_mailbox, 2 * h->mailbox_stride)) ||
-        (rc = alloc(&h->d_scratch, nS))) {
+        (rc = alloc(&h->d_scratch, nS)) || (rc = alloc(&h->d_qw, nW))) {
         free_handle(h);
         return rc;
     }
@@ -368,6 +535,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     CP(h->d_Q, Q.data(), sizeof(float) * Q.size());
     CP(h->d_shift, V.data(), sizeof(float) * V.size());
     CP(h->d_tab, TB.data(), sizeof(float) * TB.size());
+    if (nW > 0) CP(h->d_qw, QW.data(), sizeof(float) * QW.size());
 #undef CP
     if ((rc = launch(h, 0, 0, 1, nullptr)) != EMTF_OK) { free_handle(h); return rc; }
     e = hipStreamSynchronize(nullptr);
@@ -385,7 +553,7 @@ int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
     const int TI = h->p.transfer_interval;
     const bool xfer = TI > 0 && h->T > 1 && h->p.transfer_count > 0;
     const int g_end = h->gen + generations;
-    const int PI = h->polish_waves > 0 ? h->p.polish_interval : 0;
+    const int PI = h->polish ? h->p.polish_interval : 0;
     while (h->gen < g_end) {
         int g1 = g_end;
         if (xfer) {

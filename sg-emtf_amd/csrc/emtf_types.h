@@ -20,7 +20,8 @@ struct TaskDev {
     int64_t q_off;    // transform matrix Q[dim][dim], row-major (identity / diagonal materialised)
     int64_t v_off;    // shift[dim] and per-dimension table[dim]
     int64_t mb_off;   // elite mailbox [transfer_count][dim] inside one mailbox buffer
-    int64_t scr_off;  // scratch [pop][row stride]
+    int64_t scr_off;  // scratch [pop][row stride] (wide tasks: population rows, then trial rows)
+    int64_t qw_off;   // wide tasks: padded Q image [32 NT][row stride]
 };
 
 struct EngineParams {
@@ -47,6 +48,7 @@ struct EpochArgs {
     float* mailbox;
     int64_t mailbox_stride;  // elements per mailbox buffer
     float* scratch;
+    const float* Qw;   // padded Q images of the wide tasks
     EngineParams p;
     int32_t publish;       // write the task's elites to the mailbox after generation g1
     int32_t polish_waves;  // polish kernel: waves working on elites (1..4)

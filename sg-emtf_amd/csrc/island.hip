@@ -260,55 +260,151 @@ __device__ __forceinline__ int draw_excl(IdxStream& st, const Rng& rng, uint32_t
     return 0;
 }
 
-template <int KB>
-__device__ __forceinline__ void trial_de(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
-                                         const Lds& L, const Rng& rng, int gen, int ie, int h) {
-    using S = Shape<KB>;
-    const int m = g.m, il = ie % m, base = ie - il;
+// DE mates of individual ie (island-local indices) and jrand: the first three words
+// of its index stream usually give admissible mates (no rejection); otherwise the
+// sequential rejection draw is replayed, which then consumes more words.
+struct DeMates {
+    int r1, r2, r3, jrand;
+};
+
+__device__ __forceinline__ DeMates de_mates(const EpochArgs& a, const Geo& g, const Rng& rng, int gen, int ie) {
+    const int m = g.m, il = ie % m;
     const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
     const U4 b0 = rng.block((uint32_t)gen, P_DE_IDX, (uint32_t)ie, 0u, 0u);
-    const int jrand = (int)g.modD.mod(b0.w);
-    // Fast path: the first three words already give admissible mates (no rejection);
-    // otherwise replay the sequential rejection draw, which then consumes more words.
-    int r1 = (int)g.modM.mod(b0.x), r2 = (int)g.modM.mod(b0.y), r3 = (int)g.modM.mod(b0.z);
-    const bool ok = rand1 ? (m >= 4 && r1 != il && r2 != il && r2 != r1 && r3 != il && r3 != r1 && r3 != r2)
-                          : (m >= 3 && r2 != r1 && r3 != r1 && r3 != r2);
+    DeMates r;
+    r.jrand = (int)g.modD.mod(b0.w);
+    r.r1 = (int)g.modM.mod(b0.x);
+    r.r2 = (int)g.modM.mod(b0.y);
+    r.r3 = (int)g.modM.mod(b0.z);
+    const bool ok = rand1 ? (m >= 4 && r.r1 != il && r.r2 != il && r.r2 != r.r1 && r.r3 != il && r.r3 != r.r1 && r.r3 != r.r2)
+                          : (m >= 3 && r.r2 != r.r1 && r.r3 != r.r1 && r.r3 != r.r2);
     if (!ok) {
         IdxStream st{b0.x, b0.y, b0.z, 0u, 0, 3, 0};
         int ex[3];
         int nex = 0;
         if (rand1) ex[nex++] = il;
-        r1 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
-        ex[nex++] = r1;
-        r2 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
-        ex[nex++] = r2;
-        r3 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+        r.r1 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+        ex[nex++] = r.r1;
+        r.r2 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
+        ex[nex++] = r.r2;
+        r.r3 = draw_excl(st, rng, (uint32_t)gen, (uint32_t)ie, m, ex, nex);
     }
+    return r;
+}
+
+// Genes 2 s + h, s = 4 sb .. 4 sb + 3, of a DE/x/1/bin trial: mutant vb + F (v2 - v3)
+// clamped to the box, binomial crossover with the parent vi (16-bit uniforms from
+// block cb: two gene blocks per Philox block), zero past the dimension.
+__device__ __forceinline__ void de_block(float (&t)[4], const float4& vb, const float4& v2, const float4& v3,
+                                         const float4& vi, const U4& cb, int sb, int h, int jrand, int D, float F,
+                                         float lo, float hi, uint32_t tcr) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int s = 4 * sb + e, d = 2 * s + h;
+        const float diff = f4(v2, e) - f4(v3, e);
+        const float sc = F * diff;
+        float v = f4(vb, e) + sc;
+        v = clampf(v, lo, hi);
+        const uint32_t u16 = bits16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
+        const float tv = (u16 >= tcr && d != jrand) ? f4(vi, e) : v;
+        t[e] = (d < D) ? tv : 0.0f;
+    }
+}
+
+template <int KB>
+__device__ __forceinline__ void trial_de(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
+                                         const Lds& L, const Rng& rng, int gen, int ie, int h) {
+    using S = Shape<KB>;
+    const int il = ie % g.m, base = ie - il;
+    const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
+    const DeMates mt = de_mates(a, g, rng, gen, ie);
     const int off = h * S::HS;
-    const float* xb = L.X + (base + (rand1 ? r1 : il)) * S::RS + off;
-    const float* x2 = L.X + (base + r2) * S::RS + off;
-    const float* x3 = L.X + (base + r3) * S::RS + off;
+    const float* xb = L.X + (base + (rand1 ? mt.r1 : il)) * S::RS + off;
+    const float* x2 = L.X + (base + mt.r2) * S::RS + off;
+    const float* x3 = L.X + (base + mt.r3) * S::RS + off;
     const float* xi = L.X + ie * S::RS + off;
-    const float F = a.p.F, lo = td.lo, hi = td.hi;
-    const uint32_t tcr = a.p.cr_threshold;
     U4 cb{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int sb = 0; sb < KB; ++sb) {
-        // crossover uniforms: 16-bit halves, two gene blocks per Philox block
         if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
         const float4 vb = lds4(xb + 4 * sb), v2 = lds4(x2 + 4 * sb), v3 = lds4(x3 + 4 * sb), vi = lds4(xi + 4 * sb);
+        float t[4];
+        de_block(t, vb, v2, v3, vi, cb, sb, h, mt.jrand, g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int s = 4 * sb + e, d = 2 * s + h;
-            const float diff = f4(v2, e) - f4(v3, e);
-            const float sc = F * diff;
-            float v = f4(vb, e) + sc;
-            v = clampf(v, lo, hi);
-            const uint32_t u16 = bits16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
-            const float t = (u16 >= tcr && d != jrand) ? f4(vi, e) : v;
-            tr[s] = (d < g.D) ? t : 0.0f;
-        }
+        for (int e = 0; e < 4; ++e) tr[4 * sb + e] = t[e];
     }
+}
+
+// Genes 2 s + h, s = 4 sb .. 4 sb + 3, of a GA child: SBX of parents va, vb
+// (spread sign by the child's parity in its pair) then polynomial mutation,
+// each clamped to the box, zero past the dimension.
+struct GaConst {
+    float lo, hi, rng, ec, em1, em, pm;
+};
+
+__device__ __forceinline__ void ga_block(float (&t)[4], const float4& va, const float4& vb, const Rng& rng, int gen,
+                                         int ie, int sb, int h, bool odd, int D, const GaConst& c0) {
+    const U4 sbk = rng.block((uint32_t)gen, P_GA_SBX, (uint32_t)ie, (uint32_t)h, (uint32_t)sb);
+    const U4 pm0 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb));
+    const U4 pm1 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb + 1));
+    const float lo = c0.lo, hi = c0.hi, rng_ = c0.rng;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int s = 4 * sb + e, d = 2 * s + h;
+        const float p = uniform_open(pick(sbk, (uint32_t)e));
+        const float tv = (p <= 0.5f) ? (2.0f * p) : (0.5f / (1.0f - p));
+        float beta = pow_f(tv, c0.ec);
+        if (odd) beta = -beta;
+        const float t1 = 1.0f + beta;
+        const float t2 = t1 * f4(va, e);
+        const float t3 = 1.0f - beta;
+        const float t4 = t3 * f4(vb, e);
+        const float t5 = t2 + t4;
+        float c = 0.5f * t5;
+        c = clampf(c, lo, hi);
+        const U4& pb = (e < 2) ? pm0 : pm1;
+        const float r = uniform01(pick(pb, (uint32_t)((e & 1) * 2)));
+        if (r < c0.pm) {
+            const float u = uniform01(pick(pb, (uint32_t)((e & 1) * 2 + 1)));
+            float delta = 1.0f, iv1, iv2, dl;
+            const float num = c - lo;
+            dl = num / rng_;
+            iv1 = 2.0f * u;
+            iv2 = 1.0f - iv1;
+            if (u > 0.5f) {
+                delta = -1.0f;
+                const float num2 = hi - c;
+                dl = num2 / rng_;
+                const float omu = 1.0f - u;
+                iv1 = 2.0f * omu;
+                const float um = u - 0.5f;
+                iv2 = 2.0f * um;
+            }
+            const float omd = 1.0f - dl;
+            const float pw = pow_f(omd, c0.em1);
+            const float a2 = iv2 * pw;
+            const float a3 = iv1 + a2;
+            const float a4 = pow_f(a3, c0.em);
+            const float a5 = a4 - 1.0f;
+            delta = delta * a5;
+            const float step = delta * rng_;
+            c = c + step;
+            c = clampf(c, lo, hi);
+        }
+        t[e] = (d < D) ? c : 0.0f;
+    }
+}
+
+__device__ __forceinline__ GaConst ga_const(const EpochArgs& a, const TaskDev& td) {
+    GaConst c;
+    c.lo = td.lo;
+    c.hi = td.hi;
+    c.rng = td.hi - td.lo;
+    c.ec = 1.0f / (a.p.eta_c + 1.0f);
+    c.em1 = a.p.eta_m + 1.0f;
+    c.em = 1.0f / c.em1;
+    c.pm = a.p.pm;
+    return c;
 }
 
 template <int KB>
@@ -322,63 +418,15 @@ __device__ __forceinline__ void trial_ga(float (&tr)[4 * KB], const EpochArgs& a
     const int off = h * S::HS;
     const float* xa = L.X + (base + ra) * S::RS + off;
     const float* xb = L.X + (base + rb) * S::RS + off;
-    const float lo = td.lo, hi = td.hi, rng_ = hi - lo;
-    const float ec = 1.0f / (a.p.eta_c + 1.0f);
-    const float em1 = a.p.eta_m + 1.0f;
-    const float em = 1.0f / em1;
-    const float pm = a.p.pm;
+    const GaConst c0 = ga_const(a, td);
     const bool odd = (il & 1) != 0;
 #pragma unroll
     for (int sb = 0; sb < KB; ++sb) {
         const float4 va = lds4(xa + 4 * sb), vb = lds4(xb + 4 * sb);
-        const U4 sbk = rng.block((uint32_t)gen, P_GA_SBX, (uint32_t)ie, (uint32_t)h, (uint32_t)sb);
-        const U4 pm0 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb));
-        const U4 pm1 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb + 1));
+        float t[4];
+        ga_block(t, va, vb, rng, gen, ie, sb, h, odd, g.D, c0);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int s = 4 * sb + e, d = 2 * s + h;
-            const float p = uniform_open(pick(sbk, (uint32_t)e));
-            const float tv = (p <= 0.5f) ? (2.0f * p) : (0.5f / (1.0f - p));
-            float beta = pow_f(tv, ec);
-            if (odd) beta = -beta;
-            const float t1 = 1.0f + beta;
-            const float t2 = t1 * f4(va, e);
-            const float t3 = 1.0f - beta;
-            const float t4 = t3 * f4(vb, e);
-            const float t5 = t2 + t4;
-            float c = 0.5f * t5;
-            c = clampf(c, lo, hi);
-            const U4& pb = (e < 2) ? pm0 : pm1;
-            const float r = uniform01(pick(pb, (uint32_t)((e & 1) * 2)));
-            if (r < pm) {
-                const float u = uniform01(pick(pb, (uint32_t)((e & 1) * 2 + 1)));
-                float delta = 1.0f, iv1, iv2, dl;
-                const float num = c - lo;
-                dl = num / rng_;
-                iv1 = 2.0f * u;
-                iv2 = 1.0f - iv1;
-                if (u > 0.5f) {
-                    delta = -1.0f;
-                    const float num2 = hi - c;
-                    dl = num2 / rng_;
-                    const float omu = 1.0f - u;
-                    iv1 = 2.0f * omu;
-                    const float um = u - 0.5f;
-                    iv2 = 2.0f * um;
-                }
-                const float omd = 1.0f - dl;
-                const float pw = pow_f(omd, em1);
-                const float a2 = iv2 * pw;
-                const float a3 = iv1 + a2;
-                const float a4 = pow_f(a3, em);
-                const float a5 = a4 - 1.0f;
-                delta = delta * a5;
-                const float step = delta * rng_;
-                c = c + step;
-                c = clampf(c, lo, hi);
-            }
-            tr[s] = (d < g.D) ? c : 0.0f;
-        }
+        for (int e = 0; e < 4; ++e) tr[4 * sb + e] = t[e];
     }
 }
 
@@ -468,7 +516,7 @@ __device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDe
 
 // Cross-island resampling (reference SharedToGlobal + SelectFromBlocks).
 template <int KB>
-__device__ __forceinline__ void island_exchange(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+__device__ __forceinline__ void island_exchange(const EpochArgs& a, const Geo& g, const Lds& L, float* scr,
                                                 const Rng& rng, int gen, int tid) {
     constexpr int RS = Shape<KB>::RS;
     const int n = a.p.n_islands, m = g.m;
@@ -492,7 +540,6 @@ __device__ __forceinline__ void island_exchange(const EpochArgs& a, const TaskDe
         L.aux[i] = src * m + L.ord[src * m + k];
     }
     __syncthreads();
-    float* scr = a.scratch + td.scr_off;
     for (int e = tid; e < g.pop * RS; e += kThreads) {
         const int row = e / RS, c = e - row * RS;
         scr[e] = L.X[L.aux[row] * RS + c];
@@ -988,7 +1035,7 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
     const bool islands = a.p.n_islands > 1;
     for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
         if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
-            island_exchange<KB>(a, td, g, L, rng, gen, tid);
+            island_exchange<KB>(a, g, L, a.scratch + td.scr_off, rng, gen, tid);
         generation_step<KB, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
         if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
             island_migration<KB>(a, g, L, tid);
@@ -1147,8 +1194,335 @@ __global__ __launch_bounds__(2 * ObjShape<KB>::PB) void objective_kernel(ObjArgs
     }
 }
 
+#if defined(EMTF_WIDE_NT)
+// ---------------------------------------------------------------------------
+// Wide tasks: dimensions up to 32 NT whose rows (or whose population) do not fit
+// LDS.  The population rows X, the trial rows T and the task's padded Q image
+// live in global memory (a task's working set stays L2-resident while its
+// workgroup runs); LDS keeps o, the per-dimension table and the fitness and
+// rank arrays.  Trial genes are formed block by block and fed straight into the
+// MFMA k-loop as the B operand, and stored to T for the selection copy, so no
+// lane holds a whole row in registers.  Rows use the LDS path's [even | odd]
+// layout with HS = 16 NT; the gene order of every fma chain is the LDS path's,
+// with the padded k-steps past the dimension skipped (they would add +0).
+// ---------------------------------------------------------------------------
+template <int NT>
+using WShape = Shape<4 * NT>;
+
+// acc[jt] += Q rows (32 jt + col) x (4 genes of block sb of the lane's half)
+template <int NT>
+__device__ __forceinline__ void mfma_block(f16v (&acc)[NT], const float* qrow, const float4& w4, int sb) {
+    using S = WShape<NT>;
+    float4 q[NT];
+#pragma unroll
+    for (int jt = 0; jt < NT; ++jt) q[jt] = lds4(qrow + 32 * jt * S::RS + 4 * sb);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+            acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), f4(w4, e), acc[jt], 0, 0, 0);
+}
+
+__device__ __forceinline__ float4 sub4(const float4& x, const float4& o) {
+    return make_float4(x.x - o.x, x.y - o.y, x.z - o.z, x.w - o.w);
+}
+
+template <int NT>
+__device__ __forceinline__ void zero_acc(f16v (&acc)[NT]) {
+#pragma unroll
+    for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+}
+
+// f of the lane's row (its parity half at row_h) read from memory
+template <int NT, int FUNC>
+__device__ __forceinline__ float wide_eval_row(const float* row_h, const Geo& g, const Lds& L, int col, int h, int nsb) {
+    using S = WShape<NT>;
+    f16v acc[NT];
+    zero_acc<NT>(acc);
+    const float* qrow = L.Q + col * S::RS + h * S::HS;
+    const float* orow = L.o + h * S::HS;
+    for (int sb = 0; sb < nsb; ++sb) mfma_block<NT>(acc, qrow, sub4(lds4(row_h + 4 * sb), lds4(orow + 4 * sb)), sb);
+    return objective<NT, FUNC>(acc, g.D, L.tab, h);
+}
+
+template <int NT, int FUNC, int ALGO>
+__device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+                                                float* T, const Rng& rng, int gen, int tid, int nsb) {
+    using S = WShape<NT>;
+    constexpr int RS = S::RS, HS = S::HS;
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
+    const float* qrow = L.Q + col * RS + h * HS;
+    const float* orow = L.o + h * HS;
+    for (int it = wave; it < g.ntiles; it += kThreads / 64) {
+        const int i = it * 32 + col;
+        const bool valid = i < g.pop;
+        const int ie = valid ? i : 0;
+        const int il = ie % g.m, base = ie - il;
+        float* trow = T + (int64_t)i * RS + h * HS;
+        f16v acc[NT];
+        zero_acc<NT>(acc);
+        if constexpr (ALGO == A_GA_SBX_PM) {
+            const U4 mb = rng.block((uint32_t)gen, P_GA_MATE, (uint32_t)(ie & ~1), 0u, 0u);
+            const int ra = (int)g.modM.mod(pick(mb, (uint32_t)(il & 1)));
+            const int rb = (int)g.modM.mod(pick(mb, (uint32_t)((il & 1) ^ 1)));
+            const float* xa = L.X + (int64_t)(base + ra) * RS + h * HS;
+            const float* xb = L.X + (int64_t)(base + rb) * RS + h * HS;
+            const GaConst c0 = ga_const(a, td);
+            const bool odd = (il & 1) != 0;
+            for (int sb = 0; sb < nsb; ++sb) {
+                float t[4];
+                ga_block(t, lds4(xa + 4 * sb), lds4(xb + 4 * sb), rng, gen, ie, sb, h, odd, g.D, c0);
+                const float4 t4 = make_float4(t[0], t[1], t[2], t[3]);
+                *reinterpret_cast<float4*>(trow + 4 * sb) = t4;
+                mfma_block<NT>(acc, qrow, sub4(t4, lds4(orow + 4 * sb)), sb);
+            }
+        } else {
+            const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
+            const DeMates mt = de_mates(a, g, rng, gen, ie);
+            const float* xb = L.X + (int64_t)(base + (rand1 ? mt.r1 : il)) * RS + h * HS;
+            const float* x2 = L.X + (int64_t)(base + mt.r2) * RS + h * HS;
+            const float* x3 = L.X + (int64_t)(base + mt.r3) * RS + h * HS;
+            const float* xi = L.X + (int64_t)ie * RS + h * HS;
+            U4 cb{0u, 0u, 0u, 0u};
+            for (int sb = 0; sb < nsb; ++sb) {
+                if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
+                float t[4];
+                de_block(t, lds4(xb + 4 * sb), lds4(x2 + 4 * sb), lds4(x3 + 4 * sb), lds4(xi + 4 * sb), cb, sb, h,
+                         mt.jrand, g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
+                const float4 t4 = make_float4(t[0], t[1], t[2], t[3]);
+                *reinterpret_cast<float4*>(trow + 4 * sb) = t4;
+                mfma_block<NT>(acc, qrow, sub4(t4, lds4(orow + 4 * sb)), sb);
+            }
+        }
+        const float f = objective<NT, FUNC>(acc, g.D, L.tab, h);
+        const bool accept = valid && (L.fit[i] >= f);
+        if (valid && h == 0) {
+            L.aux[i] = accept ? 1 : 0;
+            L.ftmp[i] = f;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < g.pop * 2 * nsb; e += kThreads) {
+        const int row = e / (2 * nsb), blk = e - row * (2 * nsb);
+        if (L.aux[row]) {
+            const int c = (blk < nsb) ? 4 * blk : HS + 4 * (blk - nsb);
+            *reinterpret_cast<float4*>(L.X + (int64_t)row * RS + c) = lds4(T + (int64_t)row * RS + c);
+        }
+    }
+    for (int i = tid; i < g.pop; i += kThreads)
+        if (L.aux[i]) L.fit[i] = L.ftmp[i];
+    __syncthreads();
+}
+
+// Inter-task transfer for wide tasks: candidates streamed into the k-loop and
+// staged in T (row j for candidate j), accepted ones copied to their target row.
+template <int NT, int FUNC>
+__device__ __forceinline__ void wide_transfer_in(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
+                                                 float* T, const Rng& rng, int tid, int nsb) {
+    using S = WShape<NT>;
+    constexpr int RS = S::RS, HS = S::HS;
+    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0;
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
+    rank_islands(L, g.pop, g.pop, tid);
+    __syncthreads();
+    const uint32_t wd = rng.word((uint32_t)g0, P_MIG, 0u, 0u, 0u);
+    int dn = (int)(wd % (uint32_t)(a.n_tasks - 1));
+    if (dn >= (int)rng.task) dn += 1;
+    const TaskDev dd = a.tasks[dn];
+    const float* mb = a.mailbox + (int64_t)((g0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
+    const bool same_box = (dd.lo == td.lo && dd.hi == td.hi);
+    const float rng_s = td.hi - td.lo, rng_d = dd.hi - dd.lo;
+    const float* qrow = L.Q + col * RS + h * HS;
+    const float* orow = L.o + h * HS;
+    const int ntc = (k + 31) / 32;
+    for (int ct = wave; ct < ntc; ct += kThreads / 64) {
+        const int j = ct * 32 + col;
+        const bool valid = j < k;
+        const int je = valid ? j : 0;
+        const float* src = L.X + (int64_t)L.ord[je] * RS + h * HS;
+        float* crow = T + (int64_t)j * RS + h * HS;
+        f16v acc[NT];
+        zero_acc<NT>(acc);
+        for (int sb = 0; sb < nsb; ++sb) {
+            const float4 v4 = lds4(src + 4 * sb);
+            float c[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int s = 4 * sb + e, d = 2 * s + h;
+                if (d < g.D && d < dd.dim) {
+                    bool take = true;
+                    if (a.p.transfer_cr > 0.0f) {
+                        const uint32_t wu = rng.word((uint32_t)g0, P_XFER, (uint32_t)je, (uint32_t)h, (uint32_t)s);
+                        take = uniform01(wu) > a.p.transfer_cr;
+                    }
+                    if (take) {
+                        const float v = mb[(int64_t)je * dd.dim + d];
+                        if (same_box) c[e] = v;
+                        else {
+                            const float t1 = v - dd.lo;
+                            const float t2 = t1 / rng_d;
+                            const float t3 = t2 * rng_s;
+                            c[e] = td.lo + t3;
+                        }
+                    }
+                }
+            }
+            const float4 c4 = make_float4(c[0], c[1], c[2], c[3]);
+            *reinterpret_cast<float4*>(crow + 4 * sb) = c4;
+            mfma_block<NT>(acc, qrow, sub4(c4, lds4(orow + 4 * sb)), sb);
+        }
+        const float f = objective<NT, FUNC>(acc, g.D, L.tab, h);
+        if (valid) {
+            const int tgt = L.ord[g.pop - k + j];
+            if (f < L.fit[tgt]) {
+                float* dst = L.X + (int64_t)tgt * RS + h * HS;
+                for (int sb = 0; sb < nsb; ++sb) *reinterpret_cast<float4*>(dst + 4 * sb) = lds4(crow + 4 * sb);
+                if (h == 0) L.fit[tgt] = f;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int NT, int FUNC, int ALGO>
+__device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
+    using S = WShape<NT>;
+    constexpr int RS = S::RS, HS = S::HS;
+    const int tid = threadIdx.x;
+    Geo g;
+    g.D = td.dim;
+    g.Dh = (g.D + 1) >> 1;
+    g.pop = td.pop;
+    g.popP = (g.pop + 31) & ~31;
+    g.ntiles = g.popP / 32;
+    g.m = g.pop / (a.p.n_islands > 0 ? a.p.n_islands : 1);
+    g.modD.init((uint32_t)g.D);
+    g.modM.init((uint32_t)g.m);
+    const int nsb = (g.Dh + 3) >> 2;  // gene blocks per half that hold genes
+    Lds L;
+    L.X = a.scratch + td.scr_off;
+    float* T = L.X + (int64_t)g.popP * RS;
+    L.Q = const_cast<float*>(a.Qw + td.qw_off);
+    L.o = smem;
+    L.tab = L.o + RS;
+    L.fit = L.tab + 32 * NT;
+    L.ftmp = L.fit + g.popP;
+    L.ord = reinterpret_cast<int*>(L.ftmp + g.popP);
+    L.aux = L.ord + g.popP;
+    const Rng rng{a.p.seed_lo, a.p.seed_hi, (uint32_t)task};
+    const int lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
+
+    for (int e = tid; e < RS + 32 * NT; e += kThreads) smem[e] = 0.0f;
+    __syncthreads();
+    for (int d = tid; d < g.D; d += kThreads) {
+        L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
+        L.tab[d] = a.tab[td.v_off + d];
+    }
+    // rows in the padded layout, zero past the dimension
+    const float* Xg = a.X + td.x_off;
+    const float lo = td.lo, rng_ = td.hi - td.lo;
+    for (int e = tid; e < g.popP * RS; e += kThreads) {
+        const int i = e / RS, c = e - i * RS;
+        const int d = (c < HS) ? 2 * c : 2 * (c - HS) + 1;
+        float v = 0.0f;
+        if (i < g.pop && c < 2 * HS && d < g.D) {
+            if (a.init) {
+                const float u = uniform01(rng.word(0u, P_INIT, (uint32_t)i, 0u, (uint32_t)d));
+                const float t1 = u * rng_;
+                v = t1 + lo;
+            } else {
+                v = Xg[(int64_t)i * g.D + d];
+            }
+        }
+        L.X[e] = v;
+    }
+    if (!a.init)
+        for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = a.fit[td.fit_off + i];
+    __syncthreads();
+    if (a.init) {
+        for (int it = wave; it < g.ntiles; it += kThreads / 64) {
+            const int i = it * 32 + col;
+            const float f = wide_eval_row<NT, FUNC>(L.X + (int64_t)i * RS + h * HS, g, L, col, h, nsb);
+            if (i < g.pop && h == 0) L.fit[i] = f;
+        }
+        __syncthreads();
+    }
+
+    const int TI = a.p.transfer_interval;
+    const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
+    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) wide_transfer_in<NT, FUNC>(a, td, g, L, T, rng, tid, nsb);
+
+    const bool islands = a.p.n_islands > 1;
+    for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
+        if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
+            island_exchange<4 * NT>(a, g, L, T, rng, gen, tid);
+        wide_generation<NT, FUNC, ALGO>(a, td, g, L, T, rng, gen, tid, nsb);
+        if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
+            island_migration<4 * NT>(a, g, L, tid);
+    }
+
+    float* Xo = a.X + td.x_off;
+    for (int e = tid; e < g.pop * g.D; e += kThreads) {
+        const int i = e / g.D, d = e - i * g.D;
+        Xo[e] = L.X[(int64_t)i * RS + (d & 1) * HS + (d >> 1)];
+    }
+    for (int i = tid; i < g.pop; i += kThreads) a.fit[td.fit_off + i] = L.fit[i];
+    if (a.publish) {
+        __syncthreads();
+        rank_islands(L, g.pop, g.pop, tid);
+        __syncthreads();
+        const int k = a.p.transfer_count;
+        float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
+        for (int e = tid; e < k * g.D; e += kThreads) {
+            const int r = e / g.D, d = e - r * g.D;
+            mb[e] = L.X[(int64_t)L.ord[r] * RS + (d & 1) * HS + (d >> 1)];
+        }
+    }
+}
+
+template <int NT, int ALGO>
+__global__ __launch_bounds__(kThreads) void wide_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
+    extern __shared__ float smem[];
+    const int task = order[blockIdx.x];
+    const TaskDev td = a.tasks[task];
+    switch (td.func) {
+    case F_SPHERE: wide_body<NT, F_SPHERE, ALGO>(a, smem, task, td); break;
+    case F_ROSENBROCK: wide_body<NT, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
+    case F_ACKLEY: wide_body<NT, F_ACKLEY, ALGO>(a, smem, task, td); break;
+    case F_RASTRIGIN: wide_body<NT, F_RASTRIGIN, ALGO>(a, smem, task, td); break;
+    case F_GRIEWANK: wide_body<NT, F_GRIEWANK, ALGO>(a, smem, task, td); break;
+    case F_WEIERSTRASS: wide_body<NT, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
+    case F_SCHWEFEL: wide_body<NT, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
+    case F_ELLIPTIC: wide_body<NT, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
+    default: break;
+    }
+}
+#endif  // EMTF_WIDE_NT
+
 }  // namespace emtf
 
+#if defined(EMTF_WIDE_NT)
+#if !defined(EMTF_ALGO)
+#error "compile the wide kernel with -DEMTF_WIDE_NT=<row tiles> -DEMTF_ALGO=<0|2>"
+#endif
+#define EMTF_WCAT(a, b, c, d) a##b##_##c##d
+#define EMTF_WNAME(a, b, c, d) EMTF_WCAT(a, b, c, d)
+
+// Wide-task LDS: o, table, fitness, scratch fitness, ranks, flags.
+extern "C" int EMTF_WNAME(emtf_wide_lds_floats_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(int pop) {
+    const int popP = (pop + 31) & ~31;
+    return emtf::WShape<EMTF_WIDE_NT>::RS + 32 * EMTF_WIDE_NT + 4 * popP;
+}
+
+extern "C" int EMTF_WNAME(emtf_launch_wide_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
+                                                                          int n, int lds_bytes, hipStream_t stream) {
+    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO>), dim3(n), dim3(emtf::kThreads), lds_bytes,
+                       stream, *args, order);
+    return (int)hipGetLastError();
+}
+#else
 #if !defined(EMTF_KB) || !defined(EMTF_ALGO)
 #error "compile island.hip with -DEMTF_KB=<gene blocks per half> -DEMTF_ALGO=<0|2>"
 #endif
@@ -1234,3 +1608,4 @@ extern "C" int EMTF_OBJ_NAME(emtf_launch_objective_kb, EMTF_KB)(const emtf::ObjA
     return (int)(adjoint ? launch_obj_any<true>(*a, s) : launch_obj_any<false>(*a, s));
 }
 #endif
+#endif  // EMTF_WIDE_NT

@@ -56,7 +56,7 @@ def _create(descs, **kw):
 @pytest.mark.parametrize("kw, expect, msg", [
     (dict(func=9), -1, "unknown func"),
     (dict(dim=0), -1, "dim"),
-    (dict(dim=129), -4, "dim 129"),
+    (dict(dim=257), -4, "dim 257"),
     (dict(pop=33), -1, "multiple of n_islands"),
     (dict(lo=1.0, hi=1.0), -1, "bad box"),
     (dict(xform=1), -1, "rotation matrix missing"),

@@ -91,3 +91,46 @@ def test_uneven_dims_d100(oracle_mod):
     gpu.run(7)
     orc.run(7)
     _compare(gpu, orc, "uneven dims")
+
+
+# ---- wide tasks (rows in global memory) and mixed kernel buckets
+@pytest.mark.parametrize("dim", [129, 150, 200, 256])
+def test_wide_dims_de_with_transfer(oracle_mod, dim):
+    tasks = tg.benchmark_tasks(8, dim, 64, rotate=True, seed=40 + dim)
+    gpu, orc = _pair(tasks, oracle_mod, seed=3, transfer_interval=5, transfer_count=3)
+    for step in (1, 4, 6):
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"wide D={dim} gen {orc.generation}")
+
+
+def test_wide_from_lds_overflow(oracle_mod):
+    """D = 100 with pop 1024 does not fit LDS and runs on the wide kernel."""
+    tasks = tg.benchmark_tasks(4, 100, 1024, rotate=True, seed=9)
+    gpu, orc = _pair(tasks, oracle_mod, seed=8, transfer_interval=3, transfer_count=8)
+    gpu.run(4)
+    orc.run(4)
+    _compare(gpu, orc, "pop 1024 x D 100")
+
+
+def test_wide_ga_islands(oracle_mod):
+    tasks = tg.reference_tasks(4, 180, 64, seed=5)
+    params = dict(seed=2, algo=emtf.GA_SBX_PM, eta_c=2.0, eta_m=5.0, pm=0.5, n_islands=4, exchange_interval=2,
+                  migrate_interval=3, migrate_count=2)
+    gpu, orc = _pair(tasks, oracle_mod, **params)
+    gpu.run(7)
+    orc.run(7)
+    _compare(gpu, orc, "wide GA 7 gens")
+
+
+@pytest.mark.parametrize("algo", [emtf.DE_RAND_1_BIN, emtf.DE_CURRENT_1_BIN, emtf.GA_SBX_PM])
+def test_uneven_batch_buckets(oracle_mod, algo):
+    """D ~ U{10..200}, pop ~ U{32..256}: every kernel bucket at once, with islands and transfer."""
+    tasks = tg.uneven_tasks(48, seed=4)
+    params = dict(seed=6, algo=algo, n_islands=2, exchange_interval=4, migrate_interval=3, migrate_count=2,
+                  transfer_interval=5, transfer_count=2, transfer_cr=0.3)
+    gpu, orc = _pair(tasks, oracle_mod, **params)
+    for step in (5, 6):
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"uneven algo {algo} gen {orc.generation}")
