@@ -37,8 +37,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-gens", type=int, default=100, help="generations of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
-    ap.add_argument("--workload", default="plain", choices=["plain", "cvrp", "polish"],
-                    help="plain: the rotated continuous workload (default); cvrp: the CVRP GA workload; "
+    ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "cvrp", "polish"],
+                    help="plain: the rotated continuous workload (default); uneven: 4096 tasks with "
+                         "D~U{10..200}, pop~U{32..256}; cvrp: the CVRP GA workload; "
                          "polish: time to mean error 1e-6, adjoint polish vs plain DE at D=100")
     ap.add_argument("--seeds", type=int, default=10, help="polish workload: seeds 0..N-1")
     ap.add_argument("--gens", type=int, default=3000, help="polish workload: generations per run")
@@ -241,8 +242,13 @@ def main():
     from emtf import hip
     from emtf import tasks as tg
 
-    funcs = [emtf.FUNC_IDS[a.only_func]] * a.tasks if a.only_func else None
-    tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed, funcs=funcs)
+    uneven = a.workload == "uneven"
+    if uneven:
+        a.tasks = a.tasks if a.tasks != 1024 else 4096
+        tasks = tg.uneven_tasks(a.tasks, seed=a.seed)
+    else:
+        funcs = [emtf.FUNC_IDS[a.only_func]] * a.tasks if a.only_func else None
+        tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed, funcs=funcs)
     params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval,
                   transfer_count=a.elites, seed=a.seed)
     eng = emtf.Engine(tasks, **params)
@@ -264,7 +270,7 @@ def main():
     task_gens = a.tasks * a.interval * a.steps
     value = task_gens / (gpu_ms / 1e3)
 
-    flop_per_launch = 2.0 * a.pop * a.dim * a.dim * a.tasks * a.interval
+    flop_per_launch = 2.0 * sum(t["pop"] * t["dim"] ** 2 for t in tasks) * a.interval
     avg_launch_s = (gpu_ms / a.steps) / 1e3
     achieved = flop_per_launch / avg_launch_s / 1e12
     best, _ = eng.best()
@@ -272,7 +278,7 @@ def main():
 
     traffic = None
     tp = os.path.join(ROOT, "profiles", "island_epoch_traffic.json")
-    if not a.only_func and (a.tasks, a.dim, a.pop, a.interval) == (1024, 50, 128, 20) and os.path.exists(tp):
+    if not uneven and not a.only_func and (a.tasks, a.dim, a.pop, a.interval) == (1024, 50, 128, 20) and os.path.exists(tp):
         traffic = json.load(open(tp))["hbm_bytes_per_launch"]
 
     cpu = None
@@ -283,14 +289,18 @@ def main():
         "metric": METRIC, "value": value, "unit": "task-generations/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": gpu_ms / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"plain GPU: {a.tasks} rotated tasks{' (all ' + a.only_func + ')' if a.only_func else ''}, D={a.dim}, pop={a.pop}, DE/rand/1/bin "
-                               f"F=0.5 CR=0.9, {a.elites} elites every {a.interval} gens, "
+        "config": {"workload": (f"batched uneven: {a.tasks} rotated tasks, D~U{{10..200}}, pop~U{{32..256}} rounded to 32"
+                                if uneven else
+                                f"plain GPU: {a.tasks} rotated tasks{' (all ' + a.only_func + ')' if a.only_func else ''}, D={a.dim}, pop={a.pop}")
+                               + f", DE/rand/1/bin F=0.5 CR=0.9, {a.elites} elites every {a.interval} gens, "
                                f"{a.interval * a.steps} timed generations",
-                   "tasks": a.tasks, "dim": a.dim, "pop": a.pop, "generations": a.interval * a.steps,
+                   "tasks": a.tasks, "dim": "U{10..200}" if uneven else a.dim, "pop": "U{32..256}" if uneven else a.pop,
+                   "generations": a.interval * a.steps,
                    "generations_per_step": a.interval, "parallelism": "tasks (1 GPU)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": traffic,
-                     "algorithmic": "2*pop*D^2 FLOP per task-generation (rotation GEMM)",
+                     "algorithmic": "2*pop*D^2 FLOP per task-generation (rotation GEMM)"
+                                    + (", summed over tasks" if uneven else ""),
                      "avg_launch_ms": gpu_ms / a.steps},
         "cpu_baseline": cpu,
         "wall_s": wall,

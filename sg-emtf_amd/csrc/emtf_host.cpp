@@ -449,23 +449,47 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
             b.polish_lds_bytes = 4 * plds(max_pop, w, cpe);
         }
     }
-    // buckets most expensive first, tasks inside a bucket most expensive first
-    std::vector<int> border(h->buckets.size());
-    for (size_t k = 0; k < border.size(); ++k) border[k] = (int)k;
-    std::stable_sort(border.begin(), border.end(), [&](int x, int y) { return h->buckets[x].cost > h->buckets[y].cost; });
+    // Launch units: each bucket's tasks, most expensive first.  With several buckets, one
+    // carrying more than a quarter of the batch's cost is dealt round-robin (in its cost
+    // order) into units of about that size, so that the caller's stream and up to three
+    // more can be balanced; units go to the least-loaded stream, most expensive first.
+    double total = 0.0;
+    for (const Bucket& b : h->buckets) total += b.cost;
     std::vector<int> order;
     order.reserve(n_tasks);
-    std::vector<Bucket> sorted;
-    for (size_t r = 0; r < border.size(); ++r) {
-        Bucket b = h->buckets[border[r]];
-        b.offset = (int)order.size();
-        b.stream = (int)(r % 4);  // the caller's stream and up to three more
+    std::vector<Bucket> units;
+    for (size_t r = 0; r < h->buckets.size(); ++r) {
+        std::vector<int> ts;
         for (int t = 0; t < n_tasks; ++t)
-            if (bucket_of[t] == border[r]) order.push_back(t);
-        std::stable_sort(order.begin() + b.offset, order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-        sorted.push_back(b);
+            if (bucket_of[t] == (int)r) ts.push_back(t);
+        std::stable_sort(ts.begin(), ts.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+        int k = 1;
+        if (h->buckets.size() > 1) k = (int)std::ceil(h->buckets[r].cost / (total / 4.0) - 1e-9);
+        k = std::max(1, std::min(k, (int)ts.size()));
+        for (int u = 0; u < k; ++u) {
+            Bucket ub = h->buckets[r];
+            ub.offset = (int)order.size();
+            ub.count = 0;
+            ub.cost = 0.0;
+            for (size_t i = u; i < ts.size(); i += k) {
+                order.push_back(ts[i]);
+                ub.cost += cost[ts[i]];
+                ++ub.count;
+            }
+            units.push_back(ub);
+        }
     }
-    h->buckets = sorted;
+    std::stable_sort(units.begin(), units.end(), [](const Bucket& x, const Bucket& y) { return x.cost > y.cost; });
+    double load[4] = {0.0, 0.0, 0.0, 0.0};
+    const int n_streams = std::min<int>(4, (int)units.size());
+    for (Bucket& u : units) {
+        int s = 0;
+        for (int q = 1; q < n_streams; ++q)
+            if (load[q] < load[s]) s = q;
+        u.stream = s;
+        load[s] += u.cost;
+    }
+    h->buckets = units;
     const int n_side = std::min<int>(3, (int)h->buckets.size() - 1);
     for (int k = 0; k < n_side; ++k) {
         hipStream_t s = nullptr;
