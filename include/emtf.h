@@ -67,7 +67,14 @@ typedef struct emtf_params {
     int32_t polish_interval;    /* gradient polish period in generations (0 = off); no reference counterpart */
     int32_t polish_steps;       /* Armijo steps per polished elite */
     int32_t polish_topk;        /* elites polished per task */
+    int32_t schedule;           /* EMTF_SCHEDULE_*; no reference counterpart */
 } emtf_params;
+
+/* How emtf_run launches work.  EPOCHS (what AUTO selects): one launch per transfer
+ * interval, the kernel boundary ordering the elite mailbox.  PERSISTENT (one launch
+ * per call, the mailbox ordered by device-side completion stamps) is reserved and
+ * returns EMTF_E_UNSUPPORTED at create in this build. */
+enum { EMTF_SCHEDULE_AUTO = 0, EMTF_SCHEDULE_EPOCHS = 1, EMTF_SCHEDULE_PERSISTENT = 2 };
 
 typedef struct emtf_handle emtf_handle;
 

@@ -87,3 +87,13 @@ def test_null_arguments():
     assert L.emtf_create(None, 1, None, None) == -1
     assert L.emtf_run(None, 5, None) == -1
     assert L.emtf_destroy(None) == 0
+
+
+def test_persistent_schedule_rejected():
+    import emtf
+    from emtf import tasks as tg
+    tasks = tg.benchmark_tasks(2, 10, 32, rotate=False, seed=2)
+    with pytest.raises(emtf.EmtfError, match="persistent"):
+        emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, transfer_interval=5, transfer_count=2)
+    with pytest.raises(emtf.EmtfError, match="schedule"):
+        emtf.Engine(tasks, schedule=7)

@@ -134,3 +134,37 @@ def test_uneven_batch_buckets(oracle_mod, algo):
         gpu.run(step)
         orc.run(step)
         _compare(gpu, orc, f"uneven algo {algo} gen {orc.generation}")
+
+
+# ---- persistent schedule (one launch per run, device-side mailbox ordering)
+PERSISTENT_OFF = ("persistent schedule switched off: its first GPU run did not complete within the test timeout "
+                  "and no cause has been shown; the library rejects schedule=PERSISTENT")
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+@pytest.mark.parametrize("algo", [emtf.DE_RAND_1_BIN, emtf.GA_SBX_PM])
+def test_persistent_schedule_bit_exact(oracle_mod, algo):
+    tasks = tg.benchmark_tasks(24, 40, 64, rotate=True, seed=12)
+    params = dict(seed=4, algo=algo, n_islands=2, exchange_interval=3, migrate_interval=4, migrate_count=2,
+                  transfer_interval=5, transfer_count=3, transfer_cr=0.2)
+    per = emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, **params)
+    epo = emtf.Engine(tasks, schedule=emtf.SCHEDULE_EPOCHS, **params)
+    orc = oracle_mod.Engine(tasks, **params)
+    for step in (1, 9, 10, 23):
+        per.run(step)
+        epo.run(step)
+        orc.run(step)
+        _compare(per, orc, f"persistent gen {orc.generation}")
+        _compare(epo, orc, f"epochs gen {orc.generation}")
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+def test_persistent_many_epochs_per_call(oracle_mod):
+    """More epochs than one persistent launch holds: the call is split into chunks."""
+    tasks = tg.benchmark_tasks(6, 10, 32, rotate=False, seed=1)
+    params = dict(seed=9, transfer_interval=1, transfer_count=2)
+    per = emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, **params)
+    orc = oracle_mod.Engine(tasks, **params)
+    per.run(1030)
+    orc.run(1030)
+    _compare(per, orc, "1030 one-generation epochs")
