@@ -104,6 +104,12 @@ int emtf_set_population(emtf_handle* h, int32_t task, const float* x, const floa
  * Replaces popSort + messageTransfer (SG-EMTF.cu:462-486, :1601-1615). */
 int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_stride);
 
+/* The k best individuals of every task, ranked on the device by (fitness, index) with NaN
+ * last: fitness[T][k] and x[T][k][x_stride] (x_stride >= max dim; rows zero past each task's
+ * dim; either output may be NULL; k <= every task's pop; synchronises).  Replaces popSort +
+ * messageTransfer with BESTNUM_PERTASK = k (SG-EMTF.cu:60, :462-486, :1601-1615). */
+int emtf_get_elites(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t x_stride);
+
 int emtf_destroy(emtf_handle* h);
 
 /* ---- Objective and adjoint (device pointers, asynchronous on `stream`) -------

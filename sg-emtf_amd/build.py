@@ -44,7 +44,7 @@ def build(verbose=False, jobs=None, out=None, extra=()):
     units += [([HIPCC] + COMMON + list(extra) + [f"-DEMTF_WIDE_NT={nt}", f"-DEMTF_ALGO={al}", "-c",
                                    "-o", os.path.join(OBJ, f"island_w{nt}_a{al}.o"), os.path.join(CSRC, "island.hip")])
               for nt, al in WIDE]
-    for name in ("emtf_host.cpp", "cvrp_host.cpp", "cvrp.hip"):
+    for name in ("emtf_host.cpp", "cvrp_host.cpp", "cvrp.hip", "export.hip"):
         units.append([HIPCC] + COMMON + ["-c", "-o", os.path.join(OBJ, name.split(".")[0] + ".o"), os.path.join(CSRC, name)])
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(_run, units))
@@ -56,7 +56,7 @@ def build(verbose=False, jobs=None, out=None, extra=()):
             print(msg)
     objs = [os.path.join(OBJ, f"island_kb{kb}_a{al}.o") for kb, al in VARIANTS] + \
         [os.path.join(OBJ, f"island_w{nt}_a{al}.o") for nt, al in WIDE] + \
-        [os.path.join(OBJ, n + ".o") for n in ("emtf_host", "cvrp_host", "cvrp")]
+        [os.path.join(OBJ, n + ".o") for n in ("emtf_host", "cvrp_host", "cvrp", "export")]
     rc, msg = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
     if rc != 0:
         sys.stderr.write(msg + "\n")

@@ -46,6 +46,8 @@ EMTF_DECL_WPAIR(2) EMTF_DECL_WPAIR(4) EMTF_DECL_WPAIR(6) EMTF_DECL_WPAIR(8)
 EMTF_DECL_OBJ(2) EMTF_DECL_OBJ(4) EMTF_DECL_OBJ(7) EMTF_DECL_OBJ(8) EMTF_DECL_OBJ(12) EMTF_DECL_OBJ(13) EMTF_DECL_OBJ(16)
 
 typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
+extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const float* X, const float* fit, int k,
+                                  int stride, float* out_x, float* out_f, hipStream_t stream);
 
 static const int kKB[] = {2, 4, 7, 8, 12, 13, 16};
 
@@ -703,6 +705,29 @@ int emtf_objective_fwd(int32_t func, int32_t dim, int32_t n, const float* x, con
 int emtf_objective_adjoint(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
                            const float* gf, float* gx, void* stream) {
     return objective_call(func, dim, n, x, rot, shift, gf, nullptr, gx, stream, 1);
+}
+
+int emtf_get_elites(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t x_stride) {
+    g_err.clear();
+    if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
+    if (k < 1) return fail(EMTF_E_INVALID, "k < 1");
+    if (x && x_stride < h->max_dim) return fail(EMTF_E_INVALID, "x_stride < max dim");
+    for (const emtf::TaskDev& td : h->tasks)
+        if (k > td.pop) return fail(EMTF_E_INVALID, "k exceeds a task's population");
+    float *dx = nullptr, *df = nullptr;
+    const size_t nx = x ? (size_t)h->T * k * x_stride : 0, nf = (size_t)h->T * k;
+    hipError_t e = hipMalloc((void**)&df, nf * sizeof(float));
+    if (e == hipSuccess && x) e = hipMalloc((void**)&dx, nx * sizeof(float));
+    if (e == hipSuccess && x) e = hipMemsetAsync(dx, 0, nx * sizeof(float), h->stream);
+    if (e == hipSuccess)
+        e = (hipError_t)emtf_launch_elites(h->d_tasks, h->T, h->d_X, h->d_fit, k, x_stride, dx, df, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && fitness) e = hipMemcpy(fitness, df, nf * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && x) e = hipMemcpy(x, dx, nx * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dx);
+    (void)hipFree(df);
+    if (e != hipSuccess) return fail(EMTF_E_DEVICE, std::string("elite export: ") + hipGetErrorString(e));
+    return EMTF_OK;
 }
 
 int emtf_destroy(emtf_handle* h) {

@@ -130,6 +130,42 @@ class ConvergenceRecorder:
         return np.concatenate([np.stack(self.bests, 1), self.initial_worst[:, None]], 1).astype(np.float32)
 
 
+def write_para(path, tasks, rotations_path=None):
+    """Task parameters in the reference's para.txt format (SG-EMTF.cu:244-264): one line per
+    task of `M_j b_j` pairs per dimension, a blank line, then the objective ids.  The
+    reference's transform is z = M (x + b) with diagonal M, so b = -o and M = the task's
+    diagonal (1 for shift-only tasks).  Rotations do not fit the format: rotated tasks get
+    M = 1 here and their Q matrices are written to `rotations_path` (.npz, key "task<t>"),
+    which is then required."""
+    rots = {}
+    with open(path, "w") as fh:
+        for t, task in enumerate(tasks):
+            D = int(task["dim"])
+            o = np.asarray(task.get("shift", np.zeros(D)), np.float32)
+            xf = int(task.get("xform", 0))
+            m = np.asarray(task["diag"], np.float32) if xf == DIAG else np.ones(D, np.float32)
+            if xf == ROTATE:
+                rots[f"task{t}"] = np.asarray(task["rot"], np.float32)
+            fh.write("".join(f"{_fmt(m[j])} {_fmt(-o[j])} " for j in range(D)) + "\n")
+        fh.write("\n")
+        fh.write("".join(f"{int(task['func'])} " for task in tasks) + "\n")
+    if rots:
+        if rotations_path is None:
+            raise ValueError("rotated tasks need rotations_path for their Q matrices")
+        np.savez(rotations_path, **rots)
+
+
+def read_para(path):
+    """(M [T][D], b [T][D], objective ids [T]) from a para.txt file."""
+    with open(path) as fh:
+        blocks = fh.read().split("\n\n")
+    rows = [[float(v) for v in line.split()] for line in blocks[0].strip().split("\n")]
+    M = [np.asarray(r[0::2], np.float32) for r in rows]
+    b = [np.asarray(r[1::2], np.float32) for r in rows]
+    ids = [int(v) for v in blocks[1].split()]
+    return M, b, ids
+
+
 def _fmt(v):
     return "%g" % float(np.float32(v))
 

@@ -89,6 +89,13 @@ class Engine:
         _lib.check(_lib.lib().emtf_get_best(self._h, _fp(f), _fp(x), self.max_dim))
         return f, x
 
+    def elites(self, k):
+        """The k best individuals of every task: fitness [T, k] and x [T, k, max_dim] (zero past dim)."""
+        f = np.zeros((self.n_tasks, k), np.float32)
+        x = np.zeros((self.n_tasks, k, self.max_dim), np.float32)
+        _lib.check(_lib.lib().emtf_get_elites(self._h, int(k), _fp(f), _fp(x), self.max_dim))
+        return f, x
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().emtf_destroy(self._h)

@@ -81,3 +81,40 @@ def test_recorder_on_engine(tmp_path):
     cv.write_record(path, rows)
     curve = cv.ano(cv.read_record(path))
     assert curve[0] == pytest.approx(np.mean((rows[:, 0] - rows[:, -2]) / (rows[:, -1] - rows[:, -2])), rel=1e-4)
+
+
+def test_para_roundtrip(tmp_path):
+    tasks = tg.reference_tasks(3, 5, 32, seed=1) + tg.benchmark_tasks(2, 4, 32, rotate=True, seed=2)
+    p, r = str(tmp_path / "para.txt"), str(tmp_path / "rot.npz")
+    cv.write_para(p, tasks, rotations_path=r)
+    M, b, ids = cv.read_para(p)
+    assert ids == [t["func"] for t in tasks]
+    for t, task in enumerate(tasks):
+        o = task["shift"]
+        assert np.allclose(b[t], -o, rtol=1e-5, atol=1e-5)
+        want = task["diag"] if task["xform"] == 2 else np.ones(task["dim"])
+        assert np.allclose(M[t], want, rtol=1e-5)
+    rots = np.load(r)
+    assert np.array_equal(rots["task4"], tasks[4]["rot"])
+    with pytest.raises(ValueError):
+        cv.write_para(p, tasks)
+
+
+@pytest.mark.gpu
+def test_elite_export_matches_population_order():
+    import emtf
+    tasks = tg.uneven_tasks(12, seed=3)
+    eng = emtf.Engine(tasks, seed=5, transfer_interval=4, transfer_count=2)
+    eng.run(9)
+    k = 16
+    f, x = eng.elites(k)
+    for t, task in enumerate(tasks):
+        xp, fp = eng.population(t)
+        key = np.where(np.isnan(fp), np.inf, fp)
+        order = np.lexsort((np.arange(len(fp)), key))[:k]
+        assert np.array_equal(f[t].view(np.uint32), fp[order].view(np.uint32))
+        D = task["dim"]
+        assert np.array_equal(x[t, :, :D].view(np.uint32), xp[order].view(np.uint32))
+        assert not x[t, :, D:].any()
+    fb, xb = eng.best()
+    assert np.array_equal(fb.view(np.uint32), f[:, 0].view(np.uint32))
