@@ -42,6 +42,7 @@ def parse():
                          "D~U{10..200}, pop~U{32..256}; cvrp: the CVRP GA workload; "
                          "polish: time to mean error 1e-6, adjoint polish vs plain DE at D=100")
     ap.add_argument("--seeds", type=int, default=10, help="polish workload: seeds 0..N-1")
+    ap.add_argument("--no-tune", action="store_true", help="switch the kernel-variant / stream autotuner off")
     ap.add_argument("--gens", type=int, default=3000, help="polish workload: generations per run")
     ap.add_argument("--only-func", default=None, help="give every task this objective (per-objective timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -66,7 +67,7 @@ def cpu_baseline(tasks, params, gens, n_tasks):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    eng = oracle.Engine(tasks[:n_tasks], **params)
+    eng = oracle.Engine(tasks[:n_tasks], **{k: v for k, v in params.items() if k not in ("tune", "schedule")})
     t0 = time.perf_counter()
     eng.run(gens, threads)
     dt = time.perf_counter() - t0
@@ -250,7 +251,7 @@ def main():
         funcs = [emtf.FUNC_IDS[a.only_func]] * a.tasks if a.only_func else None
         tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed, funcs=funcs)
     params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval,
-                  transfer_count=a.elites, seed=a.seed)
+                  transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0)
     eng = emtf.Engine(tasks, **params)
     for _ in range(a.warmup):
         eng.run(a.interval)

@@ -24,7 +24,7 @@ COMMON = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off", "-f
 # (gene blocks per parity half KB: dims <= 8 KB; algorithm family: 0 = DE, 2 = GA)
 VARIANTS = [(kb, al) for kb in (2, 4, 7, 8, 12, 13, 16) for al in (0, 2)]
 # wide kernel: (32-row tiles NT: dims <= 32 NT, rows in global memory; algorithm family)
-WIDE = [(nt, al) for nt in (2, 4, 6, 8) for al in (0, 2)]
+WIDE = [(nt, al) for nt in (2, 4, 5, 6, 7, 8) for al in (0, 2)]
 
 
 def _run(cmd):

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -37,7 +38,7 @@ extern "C" int emtf_lds_floats_kb(int pop, int kb);
                                                    int lds_bytes, hipStream_t stream);                     \
     extern "C" int emtf_wide_lds_floats_nt##nt##_a##al(int pop);
 #define EMTF_DECL_WPAIR(nt) EMTF_DECL_WIDE(nt, 0) EMTF_DECL_WIDE(nt, 2)
-EMTF_DECL_WPAIR(2) EMTF_DECL_WPAIR(4) EMTF_DECL_WPAIR(6) EMTF_DECL_WPAIR(8)
+EMTF_DECL_WPAIR(2) EMTF_DECL_WPAIR(4) EMTF_DECL_WPAIR(5) EMTF_DECL_WPAIR(6) EMTF_DECL_WPAIR(7) EMTF_DECL_WPAIR(8)
 #define EMTF_DECL_OBJ(kb)                                                                                   \
     extern "C" int emtf_launch_objective_kb##kb(const emtf::ObjArgs* a, int adjoint, hipStream_t s);      \
     extern "C" int emtf_polish_lds_floats_kb##kb(int pop, int waves, int cpe);                             \
@@ -51,7 +52,7 @@ extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const
 
 static const int kKB[] = {2, 4, 7, 8, 12, 13, 16};
 
-static const int kNT[] = {2, 4, 6, 8};
+static const int kNT[] = {2, 4, 5, 6, 7, 8};
 
 static int pick_kb(int max_dim) {
     const int need = (max_dim + 7) / 8;
@@ -72,7 +73,7 @@ static launch_fn pick_wide(int nt, bool ga, int (**lds)(int)) {
         *lds = ga ? emtf_wide_lds_floats_nt##k##_a2 : emtf_wide_lds_floats_nt##k##_a0; \
         return ga ? emtf_launch_wide_nt##k##_a2 : emtf_launch_wide_nt##k##_a0;
     switch (nt) {
-        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(6) EMTF_CASE(8)
+        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(5) EMTF_CASE(6) EMTF_CASE(7) EMTF_CASE(8)
     }
 #undef EMTF_CASE
     return nullptr;
@@ -132,6 +133,13 @@ constexpr int kMaxLdsBytes = 160 * 1024;
 }  // namespace
 
 // Tasks run by one kernel variant: d_order[offset .. offset + count), most expensive first.
+struct Variant {
+    bool wide = false;
+    int geo = 0;                 // KB (LDS kernel) or NT (wide kernel)
+    launch_fn fn = nullptr;
+    int lds_bytes = 0;
+};
+
 struct Bucket {
     bool wide = false;
     int geo = 0;                 // KB (LDS path) or NT (wide path)
@@ -143,6 +151,9 @@ struct Bucket {
     int polish_waves = 0;        // 0: polish off
     int polish_cpe = 1;
     int stream = 0;              // index into the handle's streams (0 = the caller's)
+    Variant v[2];                // [0] primary, [1] alternative (fn null if none)
+    int use = 0;                 // variant launched
+    float ms[2] = {0.0f, 0.0f};  // calibration times of the variants
 };
 
 struct emtf_handle {
@@ -183,6 +194,7 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_qw);
     (void)hipFree(h->d_order);
     for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
+
     for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     delete h;
@@ -234,10 +246,11 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
         HIP_OR_FAIL(hipEventRecord(h->ev_fork, stream));
         for (hipStream_t s : h->side) HIP_OR_FAIL(hipStreamWaitEvent(s, h->ev_fork, 0));
     }
-    for (const Bucket& b : h->buckets) {
+    for (Bucket& b : h->buckets) {
         hipStream_t s = b.stream == 0 ? stream : h->side[b.stream - 1];
         a.publish = (pub && !(pol && b.polish_waves > 0)) ? 1 : 0;
-        int rc = b.fn(&a, h->d_order + b.offset, b.count, b.lds_bytes, s);
+        const Variant& var = b.v[b.use];
+        int rc = var.fn(&a, h->d_order + b.offset, b.count, var.lds_bytes, s);
         if (rc != 0) return fail(EMTF_E_DEVICE, std::string("island kernel launch: ") + hipGetErrorString((hipError_t)rc));
         if (pol && b.polish_waves > 0) {
             int (*plds)(int, int, int) = nullptr;
@@ -256,6 +269,79 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
             HIP_OR_FAIL(hipStreamWaitEvent(stream, h->ev_join[k], 0));
         }
     }
+    return EMTF_OK;
+}
+
+// Autotuner: with the batch initialised, time every launch unit alone for a few
+// generations with each of its kernel variants (the LDS kernel and the wide kernel
+// compute the same bits), keep the faster, and give the units to the streams
+// longest-first on the measured times.  Populations and fitness are restored after
+// each trial (the trials start at generation 0 and publish nothing), so the run's
+// results do not depend on the calibration.
+static int calibrate(emtf_handle* h, int64_t nX, int64_t nF) {
+    bool any_alt = false;
+    for (const Bucket& b : h->buckets) any_alt = any_alt || b.v[1].fn;
+    if (!any_alt && h->buckets.size() < 2) return EMTF_OK;
+    const int gens = 4;
+    float *bx = nullptr, *bf = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void**)&bx, sizeof(float) * (size_t)std::max<int64_t>(nX, 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&bf, sizeof(float) * (size_t)std::max<int64_t>(nF, 1));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemcpy(bx, h->d_X, sizeof(float) * (size_t)nX, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bf, h->d_fit, sizeof(float) * (size_t)nF, hipMemcpyDeviceToDevice);
+    emtf::EpochArgs a{};
+    a.tasks = h->d_tasks;
+    a.n_tasks = h->T;
+    a.g0 = 0;
+    a.g1 = gens;
+    a.X = h->d_X;
+    a.fit = h->d_fit;
+    a.Q = h->d_Q;
+    a.shift = h->d_shift;
+    a.tab = h->d_tab;
+    a.mailbox = h->d_mailbox;
+    a.mailbox_stride = h->mailbox_stride;
+    a.scratch = h->d_scratch;
+    a.Qw = h->d_qw;
+    a.p = h->p;
+    a.publish = 0;
+    for (Bucket& b : h->buckets) {
+        for (int v = 0; v < 2 && e == hipSuccess; ++v) {
+            if (!b.v[v].fn) continue;
+            e = hipEventRecord(e0, nullptr);
+            if (e == hipSuccess)
+                e = (hipError_t)b.v[v].fn(&a, h->d_order + b.offset, b.count, b.v[v].lds_bytes, nullptr);
+            if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&b.ms[v], e0, e1);
+            if (e == hipSuccess) e = hipMemcpy(h->d_X, bx, sizeof(float) * (size_t)nX, hipMemcpyDeviceToDevice);
+            if (e == hipSuccess) e = hipMemcpy(h->d_fit, bf, sizeof(float) * (size_t)nF, hipMemcpyDeviceToDevice);
+        }
+        b.use = (b.v[1].fn && b.ms[1] > 0.0f && b.ms[1] < b.ms[0]) ? 1 : 0;
+    }
+    (void)hipFree(bx);
+    (void)hipFree(bf);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(EMTF_E_DEVICE, std::string("calibration: ") + hipGetErrorString(e));
+    const int n_streams = 1 + (int)h->side.size();
+    std::stable_sort(h->buckets.begin(), h->buckets.end(),
+                     [](const Bucket& x, const Bucket& y) { return x.ms[x.use] > y.ms[y.use]; });
+    std::vector<double> load(n_streams, 0.0);
+    for (Bucket& b : h->buckets) {
+        int s = 0;
+        for (int q = 1; q < n_streams; ++q)
+            if (load[q] < load[s]) s = q;
+        b.stream = s;
+        load[s] += b.ms[b.use];
+    }
+    if (const char* lg = getenv("EMTF_TUNE_LOG"); lg && *lg == '1')
+        for (const Bucket& b : h->buckets)
+            fprintf(stderr, "emtf tune: %d tasks on stream %d: %s %d %.3f ms, alternative %s %d %.3f ms -> %s\n",
+                    b.count, b.stream, b.v[0].wide ? "wide NT" : "LDS KB", b.v[0].geo, b.ms[0],
+                    b.v[1].fn ? "wide NT" : "none", b.v[1].geo, b.ms[1], b.use ? "alternative" : "primary");
     return EMTF_OK;
 }
 
@@ -281,6 +367,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         return fail(EMTF_E_INVALID, "negative polish knob");
     if (prm->schedule < EMTF_SCHEDULE_AUTO || prm->schedule > EMTF_SCHEDULE_PERSISTENT)
         return fail(EMTF_E_INVALID, "unknown schedule");
+    if (prm->tune < 0 || prm->tune > 1) return fail(EMTF_E_INVALID, "tune must be 0 (on) or 1 (off)");
 
     int max_dim = 0;
     for (int t = 0; t < n_tasks; ++t) {
@@ -383,6 +470,37 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         ++b.count;
     }
 
+    // variants: [0] as chosen above; [1] for an LDS-kernel bucket, the wide kernel for
+    // its largest dimension (timed against [0] by the autotuner; not with polish, whose
+    // kernel needs the LDS rows)
+    const bool tune = prm->tune == 0 && !want_polish;
+    std::vector<int> alt_nt(h->buckets.size(), -1);
+    for (size_t k = 0; k < h->buckets.size(); ++k) {
+        Bucket& b = h->buckets[k];
+        b.v[0].wide = b.wide;
+        b.v[0].geo = b.geo;
+        b.v[0].fn = b.fn;
+        b.v[0].lds_bytes = b.lds_bytes;
+        if (!tune || b.wide) continue;
+        int maxd = 0, maxp = 0;
+        for (int t = 0; t < n_tasks; ++t)
+            if (bucket_of[t] == (int)k) {
+                maxd = std::max(maxd, tasks[t].dim);
+                maxp = std::max(maxp, tasks[t].pop);
+            }
+        const int nt = pick_nt(maxd);
+        int (*wl)(int) = nullptr;
+        launch_fn wf = nt > 0 ? pick_wide(nt, ga, &wl) : nullptr;
+        if (!wf) continue;
+        b.v[1].wide = true;
+        b.v[1].geo = nt;
+        b.v[1].fn = wf;
+        b.v[1].lds_bytes = 4 * wl(maxp);
+        alt_nt[k] = nt;
+    }
+    for (int t = 0; t < n_tasks; ++t)
+        if (alt_nt[bucket_of[t]] > 0) wide_nt[t] = alt_nt[bucket_of[t]];
+
     int64_t nX = 0, nF = 0, nQ = 0, nV = 0, nM = 0, nS = 0, nW = 0;
     h->tasks.resize(n_tasks);
     for (int t = 0; t < n_tasks; ++t) {
@@ -409,13 +527,14 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         nQ += (int64_t)d.dim * d.dim;
         nV += d.dim;
         nM += (int64_t)p.transfer_count * d.dim;
-        if (b.wide) {
-            const int RS = 32 * b.geo + 4;
-            nS += 2 * (int64_t)popP * RS;  // population rows, trial rows
-            nW += (int64_t)32 * b.geo * RS;
-        } else {
-            nS += (int64_t)popP * (8 * b.geo + 4);
+        // scratch for every variant the task may run with
+        int64_t scr = b.wide ? 0 : (int64_t)popP * (8 * b.geo + 4);
+        if (wide_nt[t] > 0) {
+            const int RS = 32 * wide_nt[t] + 4;
+            scr = std::max(scr, 2 * (int64_t)popP * RS);  // population rows, trial rows
+            nW += (int64_t)32 * wide_nt[t] * RS;
         }
+        nS += scr;
     }
     for (size_t k = 0; k < h->buckets.size(); ++k)
         if (h->buckets[k].lds_bytes > kMaxLdsBytes) {
@@ -494,6 +613,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         load[s] += u.cost;
     }
     h->buckets = units;
+
     const int n_side = std::min<int>(3, (int)h->buckets.size() - 1);
     for (int k = 0; k < n_side; ++k) {
         hipStream_t s = nullptr;
@@ -575,6 +695,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     if ((rc = launch(h, 0, 0, 1, nullptr)) != EMTF_OK) { free_handle(h); return rc; }
     e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, std::string("init: ") + hipGetErrorString(e)); }
+    if (tune && (rc = calibrate(h, nX, nF)) != EMTF_OK) { free_handle(h); return rc; }
     *out = h;
     return EMTF_OK;
 }

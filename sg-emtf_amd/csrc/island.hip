@@ -80,7 +80,7 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) {
 // halves accumulate the even / odd blocks of four rows in increasing j and
 // are combined once.
 // ---------------------------------------------------------------------------
-template <int NT, int FUNC>
+template <int NT, int FUNC, bool SKIP = false>
 __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const float* tab, int h) {
     float s1 = 0.0f, s2 = 0.0f;
     if constexpr (FUNC == F_ROSENBROCK) {
@@ -110,10 +110,16 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
                 if (j + 1 < D) s1 = s1 + v;
             }
     } else {
+        // SKIP: groups of eight rows past the dimension are skipped outright (a
+        // uniform branch) for the objectives with transcendental terms -- worth it
+        // where the row tiles are padded far past D (the wide kernel); rows j >= D
+        // are masked in every case
+        constexpr bool skip = SKIP && FUNC != F_SPHERE && FUNC != F_ELLIPTIC;
 #pragma unroll
         for (int jt = 0; jt < NT; ++jt)
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
+                if (skip && 32 * jt + 8 * (r >> 2) >= D) continue;
                 const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
                 const float z = acc[jt][r];
                 if constexpr (FUNC == F_SPHERE) {
@@ -1209,18 +1215,29 @@ __global__ __launch_bounds__(2 * ObjShape<KB>::PB) void objective_kernel(ObjArgs
 template <int NT>
 using WShape = Shape<4 * NT>;
 
-// acc[jt] += Q rows (32 jt + col) x (4 genes of block sb of the lane's half)
+// Q operands of gene block sb: rows 32 jt + col of the lane's half
 template <int NT>
-__device__ __forceinline__ void mfma_block(f16v (&acc)[NT], const float* qrow, const float4& w4, int sb) {
+__device__ __forceinline__ void load_q(float4 (&q)[NT], const float* qrow, int sb) {
     using S = WShape<NT>;
-    float4 q[NT];
 #pragma unroll
     for (int jt = 0; jt < NT; ++jt) q[jt] = lds4(qrow + 32 * jt * S::RS + 4 * sb);
+}
+
+// acc[jt] += Q rows (32 jt + col) x (4 genes of one block of the lane's half)
+template <int NT>
+__device__ __forceinline__ void mfma_q(f16v (&acc)[NT], const float4 (&q)[NT], const float4& w4) {
 #pragma unroll
     for (int e = 0; e < 4; ++e)
 #pragma unroll
         for (int jt = 0; jt < NT; ++jt)
             acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), f4(w4, e), acc[jt], 0, 0, 0);
+}
+
+template <int NT>
+__device__ __forceinline__ void mfma_block(f16v (&acc)[NT], const float* qrow, const float4& w4, int sb) {
+    float4 q[NT];
+    load_q<NT>(q, qrow, sb);
+    mfma_q<NT>(acc, q, w4);
 }
 
 __device__ __forceinline__ float4 sub4(const float4& x, const float4& o) {
@@ -1244,7 +1261,7 @@ __device__ __forceinline__ float wide_eval_row(const float* row_h, const Geo& g,
     const float* qrow = L.Q + col * S::RS + h * S::HS;
     const float* orow = L.o + h * S::HS;
     for (int sb = 0; sb < nsb; ++sb) mfma_block<NT>(acc, qrow, sub4(lds4(row_h + 4 * sb), lds4(orow + 4 * sb)), sb);
-    return objective<NT, FUNC>(acc, g.D, L.tab, h);
+    return objective<NT, FUNC, true>(acc, g.D, L.tab, h);
 }
 
 template <int NT, int FUNC, int ALGO>
@@ -1271,12 +1288,24 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
             const float* xb = L.X + (int64_t)(base + rb) * RS + h * HS;
             const GaConst c0 = ga_const(a, td);
             const bool odd = (il & 1) != 0;
+            float4 na = lds4(xa), nb = lds4(xb);
+            float4 nq[NT];
+            load_q<NT>(nq, qrow, 0);
             for (int sb = 0; sb < nsb; ++sb) {
+                const float4 va = na, vb = nb;
+                float4 q[NT];
+#pragma unroll
+                for (int jt = 0; jt < NT; ++jt) q[jt] = nq[jt];
+                if (sb + 1 < nsb) {
+                    na = lds4(xa + 4 * (sb + 1));
+                    nb = lds4(xb + 4 * (sb + 1));
+                    load_q<NT>(nq, qrow, sb + 1);
+                }
                 float t[4];
-                ga_block(t, lds4(xa + 4 * sb), lds4(xb + 4 * sb), rng, gen, ie, sb, h, odd, g.D, c0);
+                ga_block(t, va, vb, rng, gen, ie, sb, h, odd, g.D, c0);
                 const float4 t4 = make_float4(t[0], t[1], t[2], t[3]);
                 *reinterpret_cast<float4*>(trow + 4 * sb) = t4;
-                mfma_block<NT>(acc, qrow, sub4(t4, lds4(orow + 4 * sb)), sb);
+                mfma_q<NT>(acc, q, sub4(t4, lds4(orow + 4 * sb)));
             }
         } else {
             const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
@@ -1285,18 +1314,33 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
             const float* x2 = L.X + (int64_t)(base + mt.r2) * RS + h * HS;
             const float* x3 = L.X + (int64_t)(base + mt.r3) * RS + h * HS;
             const float* xi = L.X + (int64_t)ie * RS + h * HS;
+            // global-memory operands of block sb + 1 are loaded while block sb computes
             U4 cb{0u, 0u, 0u, 0u};
+            float4 nb = lds4(xb), n2 = lds4(x2), n3 = lds4(x3), ni = lds4(xi);
+            float4 nq[NT];
+            load_q<NT>(nq, qrow, 0);
             for (int sb = 0; sb < nsb; ++sb) {
+                const float4 vb = nb, v2 = n2, v3 = n3, vi = ni;
+                float4 q[NT];
+#pragma unroll
+                for (int jt = 0; jt < NT; ++jt) q[jt] = nq[jt];
+                if (sb + 1 < nsb) {
+                    const int o = 4 * (sb + 1);
+                    nb = lds4(xb + o);
+                    n2 = lds4(x2 + o);
+                    n3 = lds4(x3 + o);
+                    ni = lds4(xi + o);
+                    load_q<NT>(nq, qrow, sb + 1);
+                }
                 if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
                 float t[4];
-                de_block(t, lds4(xb + 4 * sb), lds4(x2 + 4 * sb), lds4(x3 + 4 * sb), lds4(xi + 4 * sb), cb, sb, h,
-                         mt.jrand, g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
+                de_block(t, vb, v2, v3, vi, cb, sb, h, mt.jrand, g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
                 const float4 t4 = make_float4(t[0], t[1], t[2], t[3]);
                 *reinterpret_cast<float4*>(trow + 4 * sb) = t4;
-                mfma_block<NT>(acc, qrow, sub4(t4, lds4(orow + 4 * sb)), sb);
+                mfma_q<NT>(acc, q, sub4(t4, lds4(orow + 4 * sb)));
             }
         }
-        const float f = objective<NT, FUNC>(acc, g.D, L.tab, h);
+        const float f = objective<NT, FUNC, true>(acc, g.D, L.tab, h);
         const bool accept = valid && (L.fit[i] >= f);
         if (valid && h == 0) {
             L.aux[i] = accept ? 1 : 0;
@@ -1373,7 +1417,7 @@ __device__ __forceinline__ void wide_transfer_in(const EpochArgs& a, const TaskD
             *reinterpret_cast<float4*>(crow + 4 * sb) = c4;
             mfma_block<NT>(acc, qrow, sub4(c4, lds4(orow + 4 * sb)), sb);
         }
-        const float f = objective<NT, FUNC>(acc, g.D, L.tab, h);
+        const float f = objective<NT, FUNC, true>(acc, g.D, L.tab, h);
         if (valid) {
             const int tgt = L.ord[g.pop - k + j];
             if (f < L.fit[tgt]) {
@@ -1482,8 +1526,18 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     }
 }
 
+// Two waves per SIMD: a wave's MFMA k-loop then overlaps another wave's trial
+// formation and objective epilogue (one wave per SIMD leaves the MFMA pipe idle
+// through both).
+#if defined(EMTF_WIDE_NT) && EMTF_WIDE_NT <= 7
+#define EMTF_WIDE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2, 2)))
+#else
+#define EMTF_WIDE_OCCUPANCY
+#endif
+
 template <int NT, int ALGO>
-__global__ __launch_bounds__(kThreads) void wide_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
+__global__ __launch_bounds__(kThreads) EMTF_WIDE_OCCUPANCY void wide_epoch_kernel(EpochArgs a,
+                                                                                  const int* __restrict__ order) {
     extern __shared__ float smem[];
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];

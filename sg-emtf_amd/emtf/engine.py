@@ -21,7 +21,7 @@ SCHEDULE_AUTO, SCHEDULE_EPOCHS, SCHEDULE_PERSISTENT = 0, 1, 2
 DEFAULT_PARAMS = dict(algo=DE_RAND_1_BIN, F=0.5, CR=0.9, eta_c=2.0, eta_m=5.0, pm=1.0, n_islands=1,
                       exchange_interval=0, migrate_interval=0, migrate_count=0, transfer_interval=0,
                       transfer_count=0, transfer_cr=0.0, seed=0, polish_interval=0, polish_steps=0, polish_topk=0,
-                      schedule=0)
+                      schedule=0, tune=0)
 
 
 def _fp(a):

@@ -168,3 +168,18 @@ def test_persistent_many_epochs_per_call(oracle_mod):
     per.run(1030)
     orc.run(1030)
     _compare(per, orc, "1030 one-generation epochs")
+
+
+@pytest.mark.parametrize("tune", [0, 1])
+def test_autotune_is_bit_exact(oracle_mod, tune):
+    """The autotuner times the LDS and the wide kernel of each bucket at create (and
+    restores the state), then keeps one; every epoch must agree with the oracle either way."""
+    tasks = tg.uneven_tasks(40, seed=8, dmax=120, pmax=160)
+    params = dict(seed=3, transfer_interval=2, transfer_count=2, n_islands=2, migrate_interval=3, migrate_count=2)
+    gpu = emtf.Engine(tasks, tune=tune, **params)
+    orc = oracle_mod.Engine(tasks, **params)
+    for step in (2, 2, 2, 2, 2, 6):
+        gpu.run(step)
+        gpu.sync()
+        orc.run(step)
+        _compare(gpu, orc, f"tune={tune} gen {orc.generation}")
