@@ -69,7 +69,8 @@ typedef struct emtf_params {
     int32_t polish_topk;        /* elites polished per task */
     int32_t schedule;           /* EMTF_SCHEDULE_*; no reference counterpart */
     int32_t tune;               /* 0: autotune kernel variants and stream balance from measured
-                                   epoch times (results are the same bits either way); 1: off */
+                                   epoch times (results are the same bits either way); 1: off;
+                                   2: always the alternative variant (testing) */
 } emtf_params;
 
 /* How emtf_run launches work.  EPOCHS (what AUTO selects): one launch per transfer

@@ -36,6 +36,8 @@ extern "C" int emtf_lds_floats_kb(int pop, int kb);
 #define EMTF_DECL_WIDE(nt, al)                                                                             \
     extern "C" int emtf_launch_wide_nt##nt##_a##al(const emtf::EpochArgs* args, const int* order, int n,   \
                                                    int lds_bytes, hipStream_t stream);                     \
+    extern "C" int emtf_launch_wide8_nt##nt##_a##al(const emtf::EpochArgs* args, const int* order, int n,  \
+                                                    int lds_bytes, hipStream_t stream);                    \
     extern "C" int emtf_wide_lds_floats_nt##nt##_a##al(int pop);
 #define EMTF_DECL_WPAIR(nt) EMTF_DECL_WIDE(nt, 0) EMTF_DECL_WIDE(nt, 2)
 EMTF_DECL_WPAIR(2) EMTF_DECL_WPAIR(4) EMTF_DECL_WPAIR(5) EMTF_DECL_WPAIR(6) EMTF_DECL_WPAIR(7) EMTF_DECL_WPAIR(8)
@@ -65,6 +67,17 @@ static int pick_nt(int dim) {
     for (int nt : kNT)
         if (32 * nt >= dim) return nt;
     return -1;
+}
+
+// the 512-thread build of the wide kernel (populations of more than four tiles)
+static launch_fn pick_wide8(int nt, bool ga) {
+#define EMTF_CASE(k) \
+    case k: return ga ? emtf_launch_wide8_nt##k##_a2 : emtf_launch_wide8_nt##k##_a0;
+    switch (nt) {
+        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(5) EMTF_CASE(6) EMTF_CASE(7) EMTF_CASE(8)
+    }
+#undef EMTF_CASE
+    return nullptr;
 }
 
 static launch_fn pick_wide(int nt, bool ga, int (**lds)(int)) {
@@ -341,7 +354,8 @@ static int calibrate(emtf_handle* h, int64_t nX, int64_t nF) {
         for (const Bucket& b : h->buckets)
             fprintf(stderr, "emtf tune: %d tasks on stream %d: %s %d %.3f ms, alternative %s %d %.3f ms -> %s\n",
                     b.count, b.stream, b.v[0].wide ? "wide NT" : "LDS KB", b.v[0].geo, b.ms[0],
-                    b.v[1].fn ? "wide NT" : "none", b.v[1].geo, b.ms[1], b.use ? "alternative" : "primary");
+                    b.v[1].fn ? (b.v[0].wide ? "wide x512 NT" : "wide NT") : "none", b.v[1].geo, b.ms[1],
+                    b.use ? "alternative" : "primary");
     return EMTF_OK;
 }
 
@@ -367,7 +381,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         return fail(EMTF_E_INVALID, "negative polish knob");
     if (prm->schedule < EMTF_SCHEDULE_AUTO || prm->schedule > EMTF_SCHEDULE_PERSISTENT)
         return fail(EMTF_E_INVALID, "unknown schedule");
-    if (prm->tune < 0 || prm->tune > 1) return fail(EMTF_E_INVALID, "tune must be 0 (on) or 1 (off)");
+    if (prm->tune < 0 || prm->tune > 2) return fail(EMTF_E_INVALID, "tune must be 0 (calibrate), 1 (off) or 2 (alternatives)");
 
     int max_dim = 0;
     for (int t = 0; t < n_tasks; ++t) {
@@ -471,9 +485,9 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     }
 
     // variants: [0] as chosen above; [1] for an LDS-kernel bucket, the wide kernel for
-    // its largest dimension (timed against [0] by the autotuner; not with polish, whose
-    // kernel needs the LDS rows)
-    const bool tune = prm->tune == 0 && !want_polish;
+    // its largest dimension, for a wide bucket, its 512-thread build (timed against [0]
+    // by the autotuner; not with polish, whose kernel needs the LDS rows)
+    const bool tune = prm->tune != 1 && !want_polish;
     std::vector<int> alt_nt(h->buckets.size(), -1);
     for (size_t k = 0; k < h->buckets.size(); ++k) {
         Bucket& b = h->buckets[k];
@@ -481,13 +495,21 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         b.v[0].geo = b.geo;
         b.v[0].fn = b.fn;
         b.v[0].lds_bytes = b.lds_bytes;
-        if (!tune || b.wide) continue;
+        if (!tune) continue;
         int maxd = 0, maxp = 0;
         for (int t = 0; t < n_tasks; ++t)
             if (bucket_of[t] == (int)k) {
                 maxd = std::max(maxd, tasks[t].dim);
                 maxp = std::max(maxp, tasks[t].pop);
             }
+        if (b.wide) {
+            // wide bucket: the 512-thread build when some population spans more than four tiles
+            if (maxp > 128) {
+                b.v[1] = b.v[0];
+                b.v[1].fn = pick_wide8(b.geo, ga);
+            }
+            continue;
+        }
         const int nt = pick_nt(maxd);
         int (*wl)(int) = nullptr;
         launch_fn wf = nt > 0 ? pick_wide(nt, ga, &wl) : nullptr;
@@ -695,7 +717,9 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     if ((rc = launch(h, 0, 0, 1, nullptr)) != EMTF_OK) { free_handle(h); return rc; }
     e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, std::string("init: ") + hipGetErrorString(e)); }
-    if (tune && (rc = calibrate(h, nX, nF)) != EMTF_OK) { free_handle(h); return rc; }
+    if (tune && prm->tune == 0 && (rc = calibrate(h, nX, nF)) != EMTF_OK) { free_handle(h); return rc; }
+    if (tune && prm->tune == 2)
+        for (Bucket& b : h->buckets) b.use = b.v[1].fn ? 1 : 0;
     *out = h;
     return EMTF_OK;
 }

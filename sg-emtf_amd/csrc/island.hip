@@ -458,8 +458,9 @@ __device__ __forceinline__ void store_half_row(float* row, const float (&x)[4 * 
 __device__ __forceinline__ float key_of(float f) { return (f != f) ? __builtin_inff() : f; }
 
 // ord[q m + rank] = i: the members of each island q ranked by (fitness, index)
+template <int NTHR = kThreads>
 __device__ __forceinline__ void rank_islands(const Lds& L, int pop, int m, int tid) {
-    for (int i = tid; i < pop; i += kThreads) {
+    for (int i = tid; i < pop; i += NTHR) {
         const int q = i / m, base = q * m, il = i - base;
         const float fi = key_of(L.fit[i]);
         int r = 0;
@@ -521,12 +522,12 @@ __device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDe
 }
 
 // Cross-island resampling (reference SharedToGlobal + SelectFromBlocks).
-template <int KB>
+template <int KB, int NTHR = kThreads>
 __device__ __forceinline__ void island_exchange(const EpochArgs& a, const Geo& g, const Lds& L, float* scr,
                                                 const Rng& rng, int gen, int tid) {
     constexpr int RS = Shape<KB>::RS;
     const int n = a.p.n_islands, m = g.m;
-    for (int q = tid; q < n; q += kThreads) {
+    for (int q = tid; q < n; q += NTHR) {
         int* arr = L.aux + q * m;
         for (int p = 0; p < m; ++p) arr[p] = p;
         for (int k = m - 1; k >= 0; --k) {
@@ -539,38 +540,38 @@ __device__ __forceinline__ void island_exchange(const EpochArgs& a, const Geo& g
         for (int p = 0; p < m; ++p) L.ord[q * m + arr[p]] = p;  // inverse permutation
     }
     __syncthreads();
-    for (int i = tid; i < g.pop; i += kThreads) {
+    for (int i = tid; i < g.pop; i += NTHR) {
         const int q = i / m, k = i - q * m;
         const uint32_t w = rng.word((uint32_t)gen, P_ISLX, (uint32_t)q, 1u, (uint32_t)k);
         const int src = (q + (int)(w % (uint32_t)n)) % n;
         L.aux[i] = src * m + L.ord[src * m + k];
     }
     __syncthreads();
-    for (int e = tid; e < g.pop * RS; e += kThreads) {
+    for (int e = tid; e < g.pop * RS; e += NTHR) {
         const int row = e / RS, c = e - row * RS;
         scr[e] = L.X[L.aux[row] * RS + c];
     }
-    for (int i = tid; i < g.pop; i += kThreads) L.ftmp[i] = L.fit[L.aux[i]];
+    for (int i = tid; i < g.pop; i += NTHR) L.ftmp[i] = L.fit[L.aux[i]];
     __syncthreads();
-    for (int e = tid; e < g.pop * RS; e += kThreads) L.X[e] = scr[e];
-    for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = L.ftmp[i];
+    for (int e = tid; e < g.pop * RS; e += NTHR) L.X[e] = scr[e];
+    for (int i = tid; i < g.pop; i += NTHR) L.fit[i] = L.ftmp[i];
     __syncthreads();
 }
 
 // Intra-task island migration (reference IslandMigration).
-template <int KB>
+template <int KB, int NTHR = kThreads>
 __device__ __forceinline__ void island_migration(const EpochArgs& a, const Geo& g, const Lds& L, int tid) {
     constexpr int RS = Shape<KB>::RS;
     const int n = a.p.n_islands, m = g.m, kk = a.p.migrate_count;
-    rank_islands(L, g.pop, m, tid);
+    rank_islands<NTHR>(L, g.pop, m, tid);
     __syncthreads();
-    for (int e = tid; e < n * kk; e += kThreads) {
+    for (int e = tid; e < n * kk; e += NTHR) {
         const int q = e / kk, j = e - q * kk, p = (q + 1) % n;
         const int src = L.ord[q * m + j], dst = L.ord[p * m + m - kk + j];
         L.aux[e] = (L.fit[src] < L.fit[dst]) ? 1 : 0;
     }
     __syncthreads();
-    for (int e = tid; e < n * kk * RS; e += kThreads) {
+    for (int e = tid; e < n * kk * RS; e += NTHR) {
         const int pr = e / RS, c = e - pr * RS;
         if (L.aux[pr]) {
             const int q = pr / kk, j = pr - q * kk, p = (q + 1) % n;
@@ -578,7 +579,7 @@ __device__ __forceinline__ void island_migration(const EpochArgs& a, const Geo& 
             L.X[dst * RS + c] = L.X[src * RS + c];
         }
     }
-    for (int e = tid; e < n * kk; e += kThreads) {
+    for (int e = tid; e < n * kk; e += NTHR) {
         if (L.aux[e]) {
             const int q = e / kk, j = e - q * kk, p = (q + 1) % n;
             const int src = L.ord[q * m + j], dst = L.ord[p * m + m - kk + j];
@@ -1264,7 +1265,7 @@ __device__ __forceinline__ float wide_eval_row(const float* row_h, const Geo& g,
     return objective<NT, FUNC, true>(acc, g.D, L.tab, h);
 }
 
-template <int NT, int FUNC, int ALGO>
+template <int NT, int FUNC, int ALGO, int WT>
 __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
                                                 float* T, const Rng& rng, int gen, int tid, int nsb) {
     using S = WShape<NT>;
@@ -1272,7 +1273,7 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
     const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
     const float* qrow = L.Q + col * RS + h * HS;
     const float* orow = L.o + h * HS;
-    for (int it = wave; it < g.ntiles; it += kThreads / 64) {
+    for (int it = wave; it < g.ntiles; it += WT / 64) {
         const int i = it * 32 + col;
         const bool valid = i < g.pop;
         const int ie = valid ? i : 0;
@@ -1348,28 +1349,28 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
         }
     }
     __syncthreads();
-    for (int e = tid; e < g.pop * 2 * nsb; e += kThreads) {
+    for (int e = tid; e < g.pop * 2 * nsb; e += WT) {
         const int row = e / (2 * nsb), blk = e - row * (2 * nsb);
         if (L.aux[row]) {
             const int c = (blk < nsb) ? 4 * blk : HS + 4 * (blk - nsb);
             *reinterpret_cast<float4*>(L.X + (int64_t)row * RS + c) = lds4(T + (int64_t)row * RS + c);
         }
     }
-    for (int i = tid; i < g.pop; i += kThreads)
+    for (int i = tid; i < g.pop; i += WT)
         if (L.aux[i]) L.fit[i] = L.ftmp[i];
     __syncthreads();
 }
 
 // Inter-task transfer for wide tasks: candidates streamed into the k-loop and
 // staged in T (row j for candidate j), accepted ones copied to their target row.
-template <int NT, int FUNC>
+template <int NT, int FUNC, int WT>
 __device__ __forceinline__ void wide_transfer_in(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
                                                  float* T, const Rng& rng, int tid, int nsb) {
     using S = WShape<NT>;
     constexpr int RS = S::RS, HS = S::HS;
     const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0;
     const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
-    rank_islands(L, g.pop, g.pop, tid);
+    rank_islands<WT>(L, g.pop, g.pop, tid);
     __syncthreads();
     const uint32_t wd = rng.word((uint32_t)g0, P_MIG, 0u, 0u, 0u);
     int dn = (int)(wd % (uint32_t)(a.n_tasks - 1));
@@ -1381,7 +1382,7 @@ __device__ __forceinline__ void wide_transfer_in(const EpochArgs& a, const TaskD
     const float* qrow = L.Q + col * RS + h * HS;
     const float* orow = L.o + h * HS;
     const int ntc = (k + 31) / 32;
-    for (int ct = wave; ct < ntc; ct += kThreads / 64) {
+    for (int ct = wave; ct < ntc; ct += WT / 64) {
         const int j = ct * 32 + col;
         const bool valid = j < k;
         const int je = valid ? j : 0;
@@ -1430,7 +1431,7 @@ __device__ __forceinline__ void wide_transfer_in(const EpochArgs& a, const TaskD
     __syncthreads();
 }
 
-template <int NT, int FUNC, int ALGO>
+template <int NT, int FUNC, int ALGO, int WT>
 __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
     using S = WShape<NT>;
     constexpr int RS = S::RS, HS = S::HS;
@@ -1458,16 +1459,16 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     const Rng rng{a.p.seed_lo, a.p.seed_hi, (uint32_t)task};
     const int lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
 
-    for (int e = tid; e < RS + 32 * NT; e += kThreads) smem[e] = 0.0f;
+    for (int e = tid; e < RS + 32 * NT; e += WT) smem[e] = 0.0f;
     __syncthreads();
-    for (int d = tid; d < g.D; d += kThreads) {
+    for (int d = tid; d < g.D; d += WT) {
         L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
         L.tab[d] = a.tab[td.v_off + d];
     }
     // rows in the padded layout, zero past the dimension
     const float* Xg = a.X + td.x_off;
     const float lo = td.lo, rng_ = td.hi - td.lo;
-    for (int e = tid; e < g.popP * RS; e += kThreads) {
+    for (int e = tid; e < g.popP * RS; e += WT) {
         const int i = e / RS, c = e - i * RS;
         const int d = (c < HS) ? 2 * c : 2 * (c - HS) + 1;
         float v = 0.0f;
@@ -1483,10 +1484,10 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
         L.X[e] = v;
     }
     if (!a.init)
-        for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = a.fit[td.fit_off + i];
+        for (int i = tid; i < g.pop; i += WT) L.fit[i] = a.fit[td.fit_off + i];
     __syncthreads();
     if (a.init) {
-        for (int it = wave; it < g.ntiles; it += kThreads / 64) {
+        for (int it = wave; it < g.ntiles; it += WT / 64) {
             const int i = it * 32 + col;
             const float f = wide_eval_row<NT, FUNC>(L.X + (int64_t)i * RS + h * HS, g, L, col, h, nsb);
             if (i < g.pop && h == 0) L.fit[i] = f;
@@ -1496,30 +1497,30 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
 
     const int TI = a.p.transfer_interval;
     const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
-    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) wide_transfer_in<NT, FUNC>(a, td, g, L, T, rng, tid, nsb);
+    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) wide_transfer_in<NT, FUNC, WT>(a, td, g, L, T, rng, tid, nsb);
 
     const bool islands = a.p.n_islands > 1;
     for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
         if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
-            island_exchange<4 * NT>(a, g, L, T, rng, gen, tid);
-        wide_generation<NT, FUNC, ALGO>(a, td, g, L, T, rng, gen, tid, nsb);
+            island_exchange<4 * NT, WT>(a, g, L, T, rng, gen, tid);
+        wide_generation<NT, FUNC, ALGO, WT>(a, td, g, L, T, rng, gen, tid, nsb);
         if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
-            island_migration<4 * NT>(a, g, L, tid);
+            island_migration<4 * NT, WT>(a, g, L, tid);
     }
 
     float* Xo = a.X + td.x_off;
-    for (int e = tid; e < g.pop * g.D; e += kThreads) {
+    for (int e = tid; e < g.pop * g.D; e += WT) {
         const int i = e / g.D, d = e - i * g.D;
         Xo[e] = L.X[(int64_t)i * RS + (d & 1) * HS + (d >> 1)];
     }
-    for (int i = tid; i < g.pop; i += kThreads) a.fit[td.fit_off + i] = L.fit[i];
+    for (int i = tid; i < g.pop; i += WT) a.fit[td.fit_off + i] = L.fit[i];
     if (a.publish) {
         __syncthreads();
-        rank_islands(L, g.pop, g.pop, tid);
+        rank_islands<WT>(L, g.pop, g.pop, tid);
         __syncthreads();
         const int k = a.p.transfer_count;
         float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
-        for (int e = tid; e < k * g.D; e += kThreads) {
+        for (int e = tid; e < k * g.D; e += WT) {
             const int r = e / g.D, d = e - r * g.D;
             mb[e] = L.X[(int64_t)L.ord[r] * RS + (d & 1) * HS + (d >> 1)];
         }
@@ -1528,28 +1529,28 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
 
 // Two waves per SIMD: a wave's MFMA k-loop then overlaps another wave's trial
 // formation and objective epilogue (one wave per SIMD leaves the MFMA pipe idle
-// through both).
-#if defined(EMTF_WIDE_NT) && EMTF_WIDE_NT <= 7
+// through both).  Workgroups of 256 threads (two per CU) or, for populations of
+// more than four 32-row tiles, 512 threads (one per CU, half the tiles per wave).
+#if defined(EMTF_WIDE_NT) && EMTF_WIDE_NT <= 6
 #define EMTF_WIDE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2, 2)))
 #else
 #define EMTF_WIDE_OCCUPANCY
 #endif
 
-template <int NT, int ALGO>
-__global__ __launch_bounds__(kThreads) EMTF_WIDE_OCCUPANCY void wide_epoch_kernel(EpochArgs a,
-                                                                                  const int* __restrict__ order) {
+template <int NT, int ALGO, int WT>
+__global__ __launch_bounds__(WT) EMTF_WIDE_OCCUPANCY void wide_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
     extern __shared__ float smem[];
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
     switch (td.func) {
-    case F_SPHERE: wide_body<NT, F_SPHERE, ALGO>(a, smem, task, td); break;
-    case F_ROSENBROCK: wide_body<NT, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
-    case F_ACKLEY: wide_body<NT, F_ACKLEY, ALGO>(a, smem, task, td); break;
-    case F_RASTRIGIN: wide_body<NT, F_RASTRIGIN, ALGO>(a, smem, task, td); break;
-    case F_GRIEWANK: wide_body<NT, F_GRIEWANK, ALGO>(a, smem, task, td); break;
-    case F_WEIERSTRASS: wide_body<NT, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
-    case F_SCHWEFEL: wide_body<NT, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
-    case F_ELLIPTIC: wide_body<NT, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
+    case F_SPHERE: wide_body<NT, F_SPHERE, ALGO, WT>(a, smem, task, td); break;
+    case F_ROSENBROCK: wide_body<NT, F_ROSENBROCK, ALGO, WT>(a, smem, task, td); break;
+    case F_ACKLEY: wide_body<NT, F_ACKLEY, ALGO, WT>(a, smem, task, td); break;
+    case F_RASTRIGIN: wide_body<NT, F_RASTRIGIN, ALGO, WT>(a, smem, task, td); break;
+    case F_GRIEWANK: wide_body<NT, F_GRIEWANK, ALGO, WT>(a, smem, task, td); break;
+    case F_WEIERSTRASS: wide_body<NT, F_WEIERSTRASS, ALGO, WT>(a, smem, task, td); break;
+    case F_SCHWEFEL: wide_body<NT, F_SCHWEFEL, ALGO, WT>(a, smem, task, td); break;
+    case F_ELLIPTIC: wide_body<NT, F_ELLIPTIC, ALGO, WT>(a, smem, task, td); break;
     default: break;
     }
 }
@@ -1572,8 +1573,15 @@ extern "C" int EMTF_WNAME(emtf_wide_lds_floats_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(i
 
 extern "C" int EMTF_WNAME(emtf_launch_wide_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
                                                                           int n, int lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO>), dim3(n), dim3(emtf::kThreads), lds_bytes,
-                       stream, *args, order);
+    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 256>), dim3(n), dim3(256), lds_bytes, stream,
+                       *args, order);
+    return (int)hipGetLastError();
+}
+
+extern "C" int EMTF_WNAME(emtf_launch_wide8_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
+                                                                           int n, int lds_bytes, hipStream_t stream) {
+    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512>), dim3(n), dim3(512), lds_bytes, stream,
+                       *args, order);
     return (int)hipGetLastError();
 }
 #else

@@ -26,7 +26,7 @@ def _pair(tasks, oracle_mod, **params):
     print(f"case: {len(tasks)} tasks dims {sorted(set(t['dim'] for t in tasks))} "
           f"pops {sorted(set(t['pop'] for t in tasks))} params {params}", flush=True)
     gpu = emtf.Engine(tasks, **params)
-    orc = oracle_mod.Engine(tasks, **params)
+    orc = oracle_mod.Engine(tasks, **{k: v for k, v in params.items() if k not in ("tune", "schedule")})
     return gpu, orc
 
 
@@ -170,11 +170,11 @@ def test_persistent_many_epochs_per_call(oracle_mod):
     _compare(per, orc, "1030 one-generation epochs")
 
 
-@pytest.mark.parametrize("tune", [0, 1])
+@pytest.mark.parametrize("tune", [0, 1, 2])
 def test_autotune_is_bit_exact(oracle_mod, tune):
     """The autotuner times the LDS and the wide kernel of each bucket at create (and
     restores the state), then keeps one; every epoch must agree with the oracle either way."""
-    tasks = tg.uneven_tasks(40, seed=8, dmax=120, pmax=160)
+    tasks = tg.uneven_tasks(40, seed=8, dmax=220, pmax=256)
     params = dict(seed=3, transfer_interval=2, transfer_count=2, n_islands=2, migrate_interval=3, migrate_count=2)
     gpu = emtf.Engine(tasks, tune=tune, **params)
     orc = oracle_mod.Engine(tasks, **params)
@@ -183,3 +183,14 @@ def test_autotune_is_bit_exact(oracle_mod, tune):
         gpu.sync()
         orc.run(step)
         _compare(gpu, orc, f"tune={tune} gen {orc.generation}")
+
+
+def test_wide_512_threads_ga(oracle_mod):
+    """The 512-thread wide build (tune=2 selects it for populations above four tiles)."""
+    tasks = tg.benchmark_tasks(6, 170, 224, rotate=True, seed=13)
+    params = dict(seed=5, algo=emtf.GA_SBX_PM, n_islands=4, exchange_interval=2, migrate_interval=3,
+                  migrate_count=3, transfer_interval=3, transfer_count=4)
+    gpu, orc = _pair(tasks, oracle_mod, tune=2, **params)
+    gpu.run(7)
+    orc.run(7)
+    _compare(gpu, orc, "wide x512 GA")
