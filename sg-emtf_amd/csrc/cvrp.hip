@@ -22,6 +22,7 @@ namespace emtf {
 
 constexpr uint32_t P_CV_INIT = 16, P_CV_GEN = 17;
 constexpr int kRing = 64;  // split labels kept per lane; routes hold < 64 customers
+constexpr int kSplitBlock = 4;  // steps of the split's inner loop per round of loads (4 beat 6 and 8)
 
 struct CvLds {
     uint8_t* P;     // popP x NP tours
@@ -42,6 +43,12 @@ __host__ __device__ inline int cvrp_lds_bytes(int n, int popP) {
 }
 
 // Prins split of the lane's tour t (LDS row); labels in the lane's ring column.
+// The pull loop over route starts i = j-1, j-2, ... runs in blocks of kSplitBlock
+// steps: the block's customer ids are loaded, then their demands, distances and labels, then
+// the steps are applied in order until the load exceeds the capacity -- two rounds
+// of LDS latency per block, with the sums and the minimum formed exactly as one
+// step at a time.  Loads past the stopping step stay in bounds (position < 0 reads
+// the depot; ring slots wrap) and are not used.
 __device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvLds& L, int S, int cap, int lane,
                                               int popP) {
     L.V[0 * popP + lane] = 0;
@@ -50,15 +57,35 @@ __device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvL
         int load = 0, path = 0;
         const int last = t[j - 1];
         const int back = L.dist[last * S + 0];
-        int next = last;
-        for (int i = j - 1; i >= 0; --i) {
-            const int ti = t[i];
-            load += L.dem[ti];
-            if (load > cap) break;
-            if (i < j - 1) path += L.dist[ti * S + next];
-            const int32_t c = L.V[(i & (kRing - 1)) * popP + lane] + L.dist[ti] + path + back;
-            best = c < best ? c : best;
-            next = ti;
+        int prev = last;  // customer at position i + 1 (for i = j - 1 its distance is unused)
+        bool stop = false;
+        for (int i0 = j - 1; i0 >= 0 && !stop; i0 -= kSplitBlock) {
+            int tk[kSplitBlock], dm[kSplitBlock], dd[kSplitBlock], d0[kSplitBlock];
+            int32_t v[kSplitBlock];
+#pragma unroll
+            for (int k = 0; k < kSplitBlock; ++k) tk[k] = (i0 - k >= 0) ? t[i0 - k] : 0;
+#pragma unroll
+            for (int k = 0; k < kSplitBlock; ++k) {
+                const int nx = (k == 0) ? prev : tk[k - 1];
+                dm[k] = L.dem[tk[k]];
+                dd[k] = L.dist[tk[k] * S + nx];
+                d0[k] = L.dist[tk[k]];
+                v[k] = L.V[((i0 - k) & (kRing - 1)) * popP + lane];
+            }
+#pragma unroll
+            for (int k = 0; k < kSplitBlock; ++k) {
+                const int i = i0 - k;
+                if (stop || i < 0) break;
+                load += dm[k];
+                if (load > cap) {
+                    stop = true;
+                    break;
+                }
+                if (i < j - 1) path += dd[k];
+                const int32_t c = v[k] + d0[k] + path + back;
+                best = c < best ? c : best;
+            }
+            prev = tk[kSplitBlock - 1];
         }
         L.V[(j & (kRing - 1)) * popP + lane] = best;
     }
