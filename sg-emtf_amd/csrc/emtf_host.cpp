@@ -510,6 +510,9 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
             }
             continue;
         }
+        // an LDS-kernel bucket with two or more workgroups per CU keeps its kernel: the wide
+        // kernel (rows in L2) only competes where LDS allows a single workgroup
+        if (2 * b.lds_bytes <= kMaxLdsBytes) continue;
         const int nt = pick_nt(maxd);
         int (*wl)(int) = nullptr;
         launch_fn wf = nt > 0 ? pick_wide(nt, ga, &wl) : nullptr;
