@@ -97,3 +97,46 @@ def test_persistent_schedule_rejected():
         emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, transfer_interval=5, transfer_count=2)
     with pytest.raises(emtf.EmtfError, match="schedule"):
         emtf.Engine(tasks, schedule=7)
+
+
+C_EXAMPLE = os.path.join(os.path.dirname(__file__), "c_abi", "abi_example.c")
+
+
+def _build_c_example(tmp_path):
+    import shutil
+    import subprocess
+    from emtf import _lib
+    LIB = _lib.LIB_PATH
+    gcc = shutil.which("gcc")
+    if not gcc or not os.path.exists(LIB):
+        pytest.skip("gcc or libemtf.so missing")
+    exe = str(tmp_path / "abi_example")
+    cmd = [gcc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), C_EXAMPLE, "-o", exe,
+           "-L", os.path.dirname(LIB), "-lemtf", "-Wl,-rpath," + os.path.dirname(LIB)]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+def test_c_example_compiles_against_header(tmp_path):
+    """A plain C99 program builds against include/emtf.h and links libemtf.so."""
+    _build_c_example(tmp_path)
+
+
+@pytest.mark.gpu
+def test_c_example_runs(tmp_path, oracle_mod):
+    """The C caller's results equal the oracle's for the same tasks and parameters."""
+    import subprocess
+    import numpy as np
+    exe = _build_c_example(tmp_path)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120).stdout.split()
+    assert out[0] == "ok", out
+    T, D = 16, 20
+    tasks = []
+    for t in range(T):
+        sh = np.array([((t * 7 + d * 3) % 41) - 20.0 for d in range(D)], np.float32)
+        tasks.append(dict(func=3 if t % 2 else 0, dim=D, pop=64, xform=0, lo=-100.0, hi=100.0, shift=sh))
+    orc = oracle_mod.Engine(tasks, seed=7, F=0.5, CR=0.9, transfer_interval=20, transfer_count=2)
+    orc.run(200)
+    fo = [orc.population(t)[1] for t in range(2)]
+    best = [np.float32(np.min(f)) for f in fo]
+    assert [float(v) for v in out[1:3]] == [float(b) for b in best]
