@@ -24,12 +24,13 @@ constexpr uint32_t P_CV_INIT = 16, P_CV_GEN = 17;
 constexpr int kRing = 64;  // split labels kept per lane; routes hold < 64 customers
 constexpr int kSplitBlock = 4;  // steps of the split's inner loop per round of loads (4 beat 6 and 8)
 
+template <typename LabelT>
 struct CvLds {
     uint8_t* P;     // popP x NP tours
     uint8_t* C;     // popP x NP children
     uint8_t* dist;  // (n+1) x DS
     uint8_t* dem;   // DS
-    int32_t* V;     // kRing x popP split labels
+    LabelT* V;      // kRing x popP split labels (uint16 when every label of the batch fits)
     uint32_t* used; // 8 x popP OX bitmaps
     int32_t* cost;  // popP
     int32_t* ccost; // popP
@@ -37,9 +38,9 @@ struct CvLds {
     int32_t* aux;   // popP
 };
 
-__host__ __device__ inline int cvrp_lds_bytes(int n, int popP) {
+__host__ __device__ inline int cvrp_lds_bytes(int n, int popP, int label_bytes) {
     const int NP = (n + 3) & ~3, DS = (n + 1 + 3) & ~3;
-    return 2 * popP * NP + (n + 1) * DS + DS + kRing * popP * 4 + 8 * popP * 4 + 4 * popP * 4;
+    return 2 * popP * NP + (n + 1) * DS + DS + kRing * popP * label_bytes + 8 * popP * 4 + 4 * popP * 4;
 }
 
 // Prins split of the lane's tour t (LDS row); labels in the lane's ring column.
@@ -49,7 +50,8 @@ __host__ __device__ inline int cvrp_lds_bytes(int n, int popP) {
 // of LDS latency per block, with the sums and the minimum formed exactly as one
 // step at a time.  Loads past the stopping step stay in bounds (position < 0 reads
 // the depot; ring slots wrap) and are not used.
-__device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvLds& L, int S, int cap, int lane,
+template <typename LabelT>
+__device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvLds<LabelT>& L, int S, int cap, int lane,
                                               int popP) {
     L.V[0 * popP + lane] = 0;
     for (int j = 1; j <= n; ++j) {
@@ -70,7 +72,7 @@ __device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvL
                 dm[k] = L.dem[tk[k]];
                 dd[k] = L.dist[tk[k] * S + nx];
                 d0[k] = L.dist[tk[k]];
-                v[k] = L.V[((i0 - k) & (kRing - 1)) * popP + lane];
+                v[k] = (int32_t)L.V[((i0 - k) & (kRing - 1)) * popP + lane];
             }
 #pragma unroll
             for (int k = 0; k < kSplitBlock; ++k) {
@@ -87,13 +89,14 @@ __device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvL
             }
             prev = tk[kSplitBlock - 1];
         }
-        L.V[(j & (kRing - 1)) * popP + lane] = best;
+        L.V[(j & (kRing - 1)) * popP + lane] = (LabelT)best;
     }
-    return L.V[(n & (kRing - 1)) * popP + lane];
+    return (int32_t)L.V[(n & (kRing - 1)) * popP + lane];
 }
 
+template <typename LabelT>
 __device__ __forceinline__ void make_child(const uint8_t* p1, const uint8_t* p2, uint8_t* child, int n, int a, int b,
-                                           int mut, int c, int d, const CvLds& L, int lane, int popP) {
+                                           int mut, int c, int d, const CvLds<LabelT>& L, int lane, int popP) {
     uint32_t* used = L.used + lane;
 #pragma unroll
     for (int w = 0; w < 8; ++w) used[w * popP] = 0u;
@@ -127,7 +130,8 @@ __device__ __forceinline__ void make_child(const uint8_t* p1, const uint8_t* p2,
     }
 }
 
-__device__ __forceinline__ void rank_cost(const CvLds& L, int pop, int m, int tid, int nthreads) {
+template <typename LabelT>
+__device__ __forceinline__ void rank_cost(const CvLds<LabelT>& L, int pop, int m, int tid, int nthreads) {
     for (int i = tid; i < pop; i += nthreads) {
         const int q = i / m, base = q * m, il = i - base;
         const int32_t fi = L.cost[i];
@@ -140,6 +144,7 @@ __device__ __forceinline__ void rank_cost(const CvLds& L, int pop, int m, int ti
     }
 }
 
+template <typename LabelT>
 __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* __restrict__ order) {
     extern __shared__ __align__(16) uint8_t cv_smem[];
     const int inst = order[blockIdx.x];
@@ -147,10 +152,10 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
     const int n = cd.n, cap = cd.cap, pop = cd.pop, S = n + 1;
     const int NP = (n + 3) & ~3, DS = (n + 1 + 3) & ~3;
     const int popP = blockDim.x, tid = threadIdx.x;
-    CvLds L;
+    CvLds<LabelT> L;
     uint8_t* p = cv_smem;
-    L.V = reinterpret_cast<int32_t*>(p);
-    p += kRing * popP * 4;
+    L.V = reinterpret_cast<LabelT*>(p);
+    p += kRing * popP * sizeof(LabelT);
     L.used = reinterpret_cast<uint32_t*>(p);
     p += 8 * popP * 4;
     L.cost = reinterpret_cast<int32_t*>(p);
@@ -262,17 +267,26 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
 
 }  // namespace emtf
 
-extern "C" int emtf_cvrp_lds_bytes(int n, int popP) { return emtf::cvrp_lds_bytes(n, popP); }
+extern "C" int emtf_cvrp_lds_bytes(int n, int popP, int label_bytes) {
+    return emtf::cvrp_lds_bytes(n, popP, label_bytes);
+}
 
-extern "C" int emtf_cvrp_launch(const emtf::CvrpArgs* args, const int* order, int threads, int lds_bytes,
-                                hipStream_t stream) {
-    static bool lds_ok = false;
-    if (!lds_ok) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&emtf::cvrp_epoch_kernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+// label_bytes 2: split labels stored as uint16 (the host checks that every label of the
+// batch, at most the cost of serving each customer alone, is below 2^16); else int32.
+extern "C" int emtf_cvrp_launch(const emtf::CvrpArgs* args, const int* order, int n_inst, int threads, int lds_bytes,
+                                int label_bytes, hipStream_t stream) {
+    static bool lds_ok[2] = {false, false};
+    const int w = label_bytes == 2 ? 0 : 1;
+    const void* k = w == 0 ? reinterpret_cast<const void*>(&emtf::cvrp_epoch_kernel<uint16_t>)
+                           : reinterpret_cast<const void*>(&emtf::cvrp_epoch_kernel<int32_t>);
+    if (!lds_ok[w]) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return (int)e;
-        lds_ok = true;
+        lds_ok[w] = true;
     }
-    hipLaunchKernelGGL(emtf::cvrp_epoch_kernel, dim3(args->n_inst), dim3(threads), lds_bytes, stream, *args, order);
+    if (w == 0)
+        hipLaunchKernelGGL(emtf::cvrp_epoch_kernel<uint16_t>, dim3(n_inst), dim3(threads), lds_bytes, stream, *args, order);
+    else
+        hipLaunchKernelGGL(emtf::cvrp_epoch_kernel<int32_t>, dim3(n_inst), dim3(threads), lds_bytes, stream, *args, order);
     return (int)hipGetLastError();
 }

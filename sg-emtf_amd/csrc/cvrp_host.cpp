@@ -11,9 +11,9 @@
 #include "../../include/emtf.h"
 #include "cvrp_types.h"
 
-extern "C" int emtf_cvrp_launch(const emtf::CvrpArgs* args, const int* order, int threads, int lds_bytes,
-                                hipStream_t stream);
-extern "C" int emtf_cvrp_lds_bytes(int n, int popP);
+extern "C" int emtf_cvrp_launch(const emtf::CvrpArgs* args, const int* order, int n_inst, int threads, int lds_bytes,
+                                int label_bytes, hipStream_t stream);
+extern "C" int emtf_cvrp_lds_bytes(int n, int popP, int label_bytes);
 extern "C" void emtf_set_error(const char* msg);
 
 namespace {
@@ -24,8 +24,19 @@ int cfail(int code, const std::string& msg) {
 constexpr int kMaxLds = 160 * 1024;
 }  // namespace
 
+// Instances are launched in groups of equal occupancy (workgroups per CU allowed by
+// their LDS footprint), each group with its own LDS size on its own stream, forked from
+// and joined back into the caller's stream per epoch.
+struct CvGroup {
+    int offset = 0, count = 0, lds = 0, stream = 0;
+};
+
 struct emtf_cvrp_handle {
-    int T = 0, gen = 0, threads = 64, lds = 0;
+    int T = 0, gen = 0, threads = 64, label_bytes = 4;
+    std::vector<CvGroup> groups;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev_join;
+    hipEvent_t ev_fork = nullptr;
     emtf::CvrpArgs a{};
     std::vector<emtf::CvrpDev> inst;
     emtf::CvrpDev* d_inst = nullptr;
@@ -45,6 +56,9 @@ static void cv_free(emtf_cvrp_handle* h) {
     (void)hipFree(h->d_cost);
     (void)hipFree(h->d_dist);
     (void)hipFree(h->d_dem);
+    for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     delete h;
 }
 
@@ -58,8 +72,22 @@ static int cv_launch(emtf_cvrp_handle* h, int g0, int g1, int init, hipStream_t 
     h->a.g0 = g0;
     h->a.g1 = g1;
     h->a.init = init;
-    const int rc = emtf_cvrp_launch(&h->a, h->d_order, h->threads, h->lds, s);
-    if (rc != 0) return cfail(EMTF_E_DEVICE, std::string("cvrp kernel launch: ") + hipGetErrorString((hipError_t)rc));
+    const bool fork = !h->side.empty();
+    if (fork) {
+        if (hipEventRecord(h->ev_fork, s) != hipSuccess) return cfail(EMTF_E_DEVICE, "hipEventRecord");
+        for (hipStream_t q : h->side)
+            if (hipStreamWaitEvent(q, h->ev_fork, 0) != hipSuccess) return cfail(EMTF_E_DEVICE, "hipStreamWaitEvent");
+    }
+    for (const CvGroup& g : h->groups) {
+        hipStream_t q = g.stream == 0 ? s : h->side[g.stream - 1];
+        const int rc = emtf_cvrp_launch(&h->a, h->d_order + g.offset, g.count, h->threads, g.lds, h->label_bytes, q);
+        if (rc != 0) return cfail(EMTF_E_DEVICE, std::string("cvrp kernel launch: ") + hipGetErrorString((hipError_t)rc));
+    }
+    if (fork) {
+        for (size_t k = 0; k < h->side.size(); ++k)
+            if (hipEventRecord(h->ev_join[k], h->side[k]) != hipSuccess || hipStreamWaitEvent(s, h->ev_join[k], 0) != hipSuccess)
+                return cfail(EMTF_E_DEVICE, "stream join");
+    }
     return EMTF_OK;
 }
 
@@ -107,9 +135,7 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
         nc += d.pop;
         nd += (int64_t)(d.n + 1) * (d.n + 1);
         ne += d.n + 1;
-        h->lds = std::max(h->lds, emtf_cvrp_lds_bytes(d.n, h->threads));
     }
-    if (h->lds > kMaxLds) { cv_free(h); return cfail(EMTF_E_UNSUPPORTED, "instance exceeds LDS"); }
     std::vector<uint8_t> dist((size_t)nd), dem((size_t)ne);
     for (int t = 0; t < n_inst; ++t) {
         const emtf_cvrp_instance& c = in[t];
@@ -123,10 +149,56 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
             }
         }
     }
-    // longest instances first
+    // 16-bit split labels when every label of the batch fits: a label never exceeds the
+    // cost of serving each customer of the prefix alone, sum_c 2 d(0, c)
+    h->label_bytes = 2;
+    for (int t = 0; t < n_inst && h->label_bytes == 2; ++t) {
+        const int S = in[t].n + 1;
+        int64_t bound = 0;
+        for (int c = 1; c < S; ++c) bound += 2 * (int64_t)dist[h->inst[t].dist_off + c];
+        if (bound >= 65536) h->label_bytes = 4;
+    }
+    // groups of equal occupancy (workgroups per CU from the LDS footprint), longest
+    // instances first inside each group
+    std::vector<int> occ(n_inst);
+    for (int t = 0; t < n_inst; ++t) {
+        const int b = emtf_cvrp_lds_bytes(in[t].n, h->threads, h->label_bytes);
+        if (b > kMaxLds) { cv_free(h); return cfail(EMTF_E_UNSUPPORTED, "instance exceeds LDS"); }
+        occ[t] = std::min(8, kMaxLds / b);
+    }
     std::vector<int> order(n_inst);
     for (int t = 0; t < n_inst; ++t) order[t] = t;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return in[x].n * in[x].pop > in[y].n * in[y].pop; });
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (occ[x] != occ[y]) return occ[x] < occ[y];
+        return in[x].n * in[x].pop > in[y].n * in[y].pop;
+    });
+    for (int r = 0; r < n_inst;) {
+        CvGroup g;
+        g.offset = r;
+        while (r < n_inst && occ[order[r]] == occ[order[g.offset]]) {
+            g.lds = std::max(g.lds, emtf_cvrp_lds_bytes(in[order[r]].n, h->threads, h->label_bytes));
+            ++r;
+        }
+        g.count = r - g.offset;
+        g.stream = (int)(h->groups.size() % 4);
+        h->groups.push_back(g);
+    }
+    const int n_side = std::min<int>(3, (int)h->groups.size() - 1);
+    for (int k = 0; k < n_side; ++k) {
+        hipStream_t q = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            cv_free(h);
+            return cfail(EMTF_E_DEVICE, "stream / event creation failed");
+        }
+        h->side.push_back(q);
+        h->ev_join.push_back(ev);
+    }
+    if (n_side > 0 && hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess) {
+        cv_free(h);
+        return cfail(EMTF_E_DEVICE, "event creation failed");
+    }
     auto ok = [&](hipError_t e) { return e == hipSuccess; };
     if (!ok(hipMalloc((void**)&h->d_inst, sizeof(emtf::CvrpDev) * n_inst)) || !ok(hipMalloc((void**)&h->d_order, sizeof(int) * n_inst)) ||
         !ok(hipMalloc((void**)&h->d_tours, (size_t)nt)) || !ok(hipMalloc((void**)&h->d_cost, sizeof(int32_t) * (size_t)nc)) ||

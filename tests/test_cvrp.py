@@ -127,3 +127,28 @@ def test_gpu_cvrp_bit_exact(oracle_mod, n_islands):
             to, co = orc.population(i)
             assert np.array_equal(cg, co), (step, i, cg[:4], co[:4])
             assert np.array_equal(tg, to), (step, i)
+
+
+@pytest.mark.gpu
+def test_gpu_cvrp_bit_exact_32bit_labels(oracle_mod):
+    """Instances whose split labels can exceed 2^16 (corner depot, spread coordinates)
+    run with 32-bit labels; results stay bit-exact."""
+    from emtf.cvrp import CvrpEngine
+    rng = np.random.default_rng(5)
+    inst = []
+    for n in (255, 240, 60):
+        xy = np.vstack([[0, 0], rng.integers(0, 181, (n, 2))]).astype(np.int32)
+        dem = np.concatenate([[0], rng.integers(1, 11, n)]).astype(np.int32)
+        inst.append(dict(n=n, capacity=50, pop=64, xy=xy, demand=dem))
+    d0 = oracle_mod.cvrp_dist_matrix(inst[0]["xy"])[0]
+    assert 2 * int(d0[1:].astype(np.int64).sum()) >= 65536
+    params = dict(p_swap=0.3, p_2opt=0.3, n_islands=2, migrate_interval=4, migrate_count=2, seed=3)
+    gpu = CvrpEngine(inst, **params)
+    orc = oracle_mod.CvrpEngine(inst, **params)
+    gpu.run(9)
+    orc.run(9)
+    for i in range(len(inst)):
+        tg, cg = gpu.population(i)
+        to, co = orc.population(i)
+        assert np.array_equal(cg, co), i
+        assert np.array_equal(tg, to), i

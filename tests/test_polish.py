@@ -87,3 +87,19 @@ def test_gpu_polish_bit_exact(oracle_mod, dim, pop, topk, steps):
         gpu.run(step)
         orc.run(step)
         _compare(gpu, orc, f"polish gen {orc.generation}")
+
+
+@pytest.mark.gpu
+def test_gpu_polish_across_buckets(oracle_mod):
+    """Polish with tasks of several row geometries (one LDS kernel bucket each, on their
+    own streams), with transfer publishing the polished elites across buckets."""
+    dims = [12, 40, 70, 100, 12, 40, 70, 100]
+    tasks = tg.benchmark_tasks(8, 0, 64, rotate=True, seed=17, dims=dims, pops=[64, 96, 64, 128, 32, 64, 96, 64])
+    params = dict(seed=2, F=0.5, CR=0.9, transfer_interval=5, transfer_count=2, polish_interval=5,
+                  polish_steps=3, polish_topk=4)
+    gpu = emtf.Engine(tasks, **params)
+    orc = oracle_mod.Engine(tasks, **params)
+    for step in (5, 10, 3):
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"polish buckets gen {orc.generation}")
