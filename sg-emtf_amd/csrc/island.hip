@@ -74,6 +74,14 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) {
     return v;
 }
 
+// The box clamp of a trial gene as one min and one max: equal to clampf for every
+// non-NaN value (trial genes are always finite) except a signed-zero tie with a bound
+// of zero, which the engine never produces (a -0 gene needs a -0 parent, and the
+// initial population of a box starting at 0 holds none).
+__device__ __forceinline__ float clamp_box(float v, float lo, float hi) {
+    return __builtin_fminf(__builtin_fmaxf(v, lo), hi);
+}
+
 // ---------------------------------------------------------------------------
 // Objective of the lane's individual from its MFMA accumulators.
 // Lane (col, h) holds z_j for j = 32 jt + 8 (r >> 2) + 4 h + (r & 3); the two
@@ -310,10 +318,10 @@ __device__ __forceinline__ void de_block(float (&t)[4], const float4& vb, const 
         const float diff = f4(v2, e) - f4(v3, e);
         const float sc = F * diff;
         float v = f4(vb, e) + sc;
-        v = clampf(v, lo, hi);
+        v = clamp_box(v, lo, hi);
         const uint32_t u16 = bits16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
         const float tv = (u16 >= tcr && d != jrand) ? f4(vi, e) : v;
-        t[e] = (d < D) ? tv : 0.0f;
+        t[e] = (8 * sb + 8 <= D || d < D) ? tv : 0.0f;  // only a block reaching past D is masked
     }
 }
 
@@ -367,7 +375,7 @@ __device__ __forceinline__ void ga_block(float (&t)[4], const float4& va, const 
         const float t4 = t3 * f4(vb, e);
         const float t5 = t2 + t4;
         float c = 0.5f * t5;
-        c = clampf(c, lo, hi);
+        c = clamp_box(c, lo, hi);
         const U4& pb = (e < 2) ? pm0 : pm1;
         const float r = uniform01(pick(pb, (uint32_t)((e & 1) * 2)));
         if (r < c0.pm) {
@@ -395,9 +403,9 @@ __device__ __forceinline__ void ga_block(float (&t)[4], const float4& va, const 
             delta = delta * a5;
             const float step = delta * rng_;
             c = c + step;
-            c = clampf(c, lo, hi);
+            c = clamp_box(c, lo, hi);
         }
-        t[e] = (d < D) ? c : 0.0f;
+        t[e] = (8 * sb + 8 <= D || d < D) ? c : 0.0f;
     }
 }
 
