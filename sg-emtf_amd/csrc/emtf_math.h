@@ -28,6 +28,10 @@ struct U4 { uint32_t x, y, z, w; };
 
 __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                      uint32_t k0, uint32_t k1) {
+    // the key schedule is re-derived per block with two scalar adds a round; without
+    // this the compiler keeps all twenty round keys live across the generation loop,
+    // where they spill and come back as VALU lane reads
+    asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
