@@ -70,7 +70,9 @@ typedef struct emtf_params {
     int32_t schedule;           /* EMTF_SCHEDULE_*; no reference counterpart */
     int32_t tune;               /* 0: autotune kernel variants and stream balance from measured
                                    epoch times (results are the same bits either way); 1: off;
-                                   2: always the alternative variant (testing) */
+                                   2: the first alternative variant where one exists (the wide
+                                   kernel, or its 512-thread build); 3: the wide kernel with Q in
+                                   LDS where it fits (2 and 3 for testing) */
 } emtf_params;
 
 /* How emtf_run launches work.  EPOCHS (what AUTO selects): one launch per transfer

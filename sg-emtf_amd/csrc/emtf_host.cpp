@@ -41,6 +41,13 @@ extern "C" int emtf_lds_floats_kb(int pop, int kb);
     extern "C" int emtf_wide_lds_floats_nt##nt##_a##al(int pop);
 #define EMTF_DECL_WPAIR(nt) EMTF_DECL_WIDE(nt, 0) EMTF_DECL_WIDE(nt, 2)
 EMTF_DECL_WPAIR(2) EMTF_DECL_WPAIR(4) EMTF_DECL_WPAIR(5) EMTF_DECL_WPAIR(6) EMTF_DECL_WPAIR(7) EMTF_DECL_WPAIR(8)
+// the wide kernel with the task's Q image in LDS (NT <= 6, 512 threads)
+#define EMTF_DECL_WIDEQ(nt, al)                                                                            \
+    extern "C" int emtf_launch_wideq_nt##nt##_a##al(const emtf::EpochArgs* args, const int* order, int n,  \
+                                                    int lds_bytes, hipStream_t stream);                    \
+    extern "C" int emtf_wideq_lds_floats_nt##nt##_a##al(int pop);
+#define EMTF_DECL_QPAIR(nt) EMTF_DECL_WIDEQ(nt, 0) EMTF_DECL_WIDEQ(nt, 2)
+EMTF_DECL_QPAIR(2) EMTF_DECL_QPAIR(4) EMTF_DECL_QPAIR(5) EMTF_DECL_QPAIR(6)
 #define EMTF_DECL_OBJ(kb)                                                                                   \
     extern "C" int emtf_launch_objective_kb##kb(const emtf::ObjArgs* a, int adjoint, hipStream_t s);      \
     extern "C" int emtf_polish_lds_floats_kb##kb(int pop, int waves, int cpe);                             \
@@ -75,6 +82,18 @@ static launch_fn pick_wide8(int nt, bool ga) {
     case k: return ga ? emtf_launch_wide8_nt##k##_a2 : emtf_launch_wide8_nt##k##_a0;
     switch (nt) {
         EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(5) EMTF_CASE(6) EMTF_CASE(7) EMTF_CASE(8)
+    }
+#undef EMTF_CASE
+    return nullptr;
+}
+
+static launch_fn pick_wideq(int nt, bool ga, int (**lds)(int)) {
+#define EMTF_CASE(k)                                                                 \
+    case k:                                                                          \
+        *lds = ga ? emtf_wideq_lds_floats_nt##k##_a2 : emtf_wideq_lds_floats_nt##k##_a0; \
+        return ga ? emtf_launch_wideq_nt##k##_a2 : emtf_launch_wideq_nt##k##_a0;
+    switch (nt) {
+        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(5) EMTF_CASE(6)
     }
 #undef EMTF_CASE
     return nullptr;
@@ -151,7 +170,10 @@ struct Variant {
     int geo = 0;                 // KB (LDS kernel) or NT (wide kernel)
     launch_fn fn = nullptr;
     int lds_bytes = 0;
+    const char* name = "";
 };
+
+constexpr int kVariants = 3;
 
 struct Bucket {
     bool wide = false;
@@ -164,9 +186,9 @@ struct Bucket {
     int polish_waves = 0;        // 0: polish off
     int polish_cpe = 1;
     int stream = 0;              // index into the handle's streams (0 = the caller's)
-    Variant v[2];                // [0] primary, [1] alternative (fn null if none)
+    Variant v[kVariants];        // [0] primary, [1] [2] alternatives (fn null if none)
     int use = 0;                 // variant launched
-    float ms[2] = {0.0f, 0.0f};  // calibration times of the variants
+    float ms[kVariants] = {};    // calibration times of the variants
 };
 
 struct emtf_handle {
@@ -293,7 +315,8 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
 // results do not depend on the calibration.
 static int calibrate(emtf_handle* h, int64_t nX, int64_t nF) {
     bool any_alt = false;
-    for (const Bucket& b : h->buckets) any_alt = any_alt || b.v[1].fn;
+    for (const Bucket& b : h->buckets)
+        for (int v = 1; v < kVariants; ++v) any_alt = any_alt || b.v[v].fn;
     if (!any_alt && h->buckets.size() < 2) return EMTF_OK;
     const int gens = 4;
     float *bx = nullptr, *bf = nullptr;
@@ -321,18 +344,26 @@ static int calibrate(emtf_handle* h, int64_t nX, int64_t nF) {
     a.p = h->p;
     a.publish = 0;
     for (Bucket& b : h->buckets) {
-        for (int v = 0; v < 2 && e == hipSuccess; ++v) {
+        // two trials per variant, the faster kept: a kernel's first launch in the process
+        // also pays for loading its code object
+        for (int v = 0; v < kVariants && e == hipSuccess; ++v) {
             if (!b.v[v].fn) continue;
-            e = hipEventRecord(e0, nullptr);
-            if (e == hipSuccess)
-                e = (hipError_t)b.v[v].fn(&a, h->d_order + b.offset, b.count, b.v[v].lds_bytes, nullptr);
-            if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            if (e == hipSuccess) e = hipEventElapsedTime(&b.ms[v], e0, e1);
-            if (e == hipSuccess) e = hipMemcpy(h->d_X, bx, sizeof(float) * (size_t)nX, hipMemcpyDeviceToDevice);
-            if (e == hipSuccess) e = hipMemcpy(h->d_fit, bf, sizeof(float) * (size_t)nF, hipMemcpyDeviceToDevice);
+            for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+                float ms = 0.0f;
+                e = hipEventRecord(e0, nullptr);
+                if (e == hipSuccess)
+                    e = (hipError_t)b.v[v].fn(&a, h->d_order + b.offset, b.count, b.v[v].lds_bytes, nullptr);
+                if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e == hipSuccess) e = hipMemcpy(h->d_X, bx, sizeof(float) * (size_t)nX, hipMemcpyDeviceToDevice);
+                if (e == hipSuccess) e = hipMemcpy(h->d_fit, bf, sizeof(float) * (size_t)nF, hipMemcpyDeviceToDevice);
+                b.ms[v] = rep == 0 ? ms : std::min(b.ms[v], ms);
+            }
         }
-        b.use = (b.v[1].fn && b.ms[1] > 0.0f && b.ms[1] < b.ms[0]) ? 1 : 0;
+        b.use = 0;
+        for (int v = 1; v < kVariants; ++v)
+            if (b.v[v].fn && b.ms[v] > 0.0f && b.ms[v] < b.ms[b.use]) b.use = v;
     }
     (void)hipFree(bx);
     (void)hipFree(bf);
@@ -351,11 +382,12 @@ static int calibrate(emtf_handle* h, int64_t nX, int64_t nF) {
         load[s] += b.ms[b.use];
     }
     if (const char* lg = getenv("EMTF_TUNE_LOG"); lg && *lg == '1')
-        for (const Bucket& b : h->buckets)
-            fprintf(stderr, "emtf tune: %d tasks on stream %d: %s %d %.3f ms, alternative %s %d %.3f ms -> %s\n",
-                    b.count, b.stream, b.v[0].wide ? "wide NT" : "LDS KB", b.v[0].geo, b.ms[0],
-                    b.v[1].fn ? (b.v[0].wide ? "wide x512 NT" : "wide NT") : "none", b.v[1].geo, b.ms[1],
-                    b.use ? "alternative" : "primary");
+        for (const Bucket& b : h->buckets) {
+            fprintf(stderr, "emtf tune: %d tasks on stream %d:", b.count, b.stream);
+            for (int v = 0; v < kVariants; ++v)
+                if (b.v[v].fn) fprintf(stderr, " [%d] %s %d %.3f ms", v, b.v[v].name, b.v[v].geo, b.ms[v]);
+            fprintf(stderr, " -> [%d]\n", b.use);
+        }
     return EMTF_OK;
 }
 
@@ -381,7 +413,8 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         return fail(EMTF_E_INVALID, "negative polish knob");
     if (prm->schedule < EMTF_SCHEDULE_AUTO || prm->schedule > EMTF_SCHEDULE_PERSISTENT)
         return fail(EMTF_E_INVALID, "unknown schedule");
-    if (prm->tune < 0 || prm->tune > 2) return fail(EMTF_E_INVALID, "tune must be 0 (calibrate), 1 (off) or 2 (alternatives)");
+    if (prm->tune < 0 || prm->tune > 3)
+        return fail(EMTF_E_INVALID, "tune must be 0 (calibrate), 1 (off), 2 or 3 (one alternative)");
 
     int max_dim = 0;
     for (int t = 0; t < n_tasks; ++t) {
@@ -485,8 +518,9 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     }
 
     // variants: [0] as chosen above; [1] for an LDS-kernel bucket, the wide kernel for
-    // its largest dimension, for a wide bucket, its 512-thread build (timed against [0]
-    // by the autotuner; not with polish, whose kernel needs the LDS rows)
+    // its largest dimension, for a wide bucket, its 512-thread build; [2] the wide kernel
+    // with Q in LDS where it fits (timed against [0] by the autotuner; not with polish,
+    // whose kernel needs the LDS rows)
     const bool tune = prm->tune != 1 && !want_polish;
     std::vector<int> alt_nt(h->buckets.size(), -1);
     for (size_t k = 0; k < h->buckets.size(); ++k) {
@@ -495,6 +529,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         b.v[0].geo = b.geo;
         b.v[0].fn = b.fn;
         b.v[0].lds_bytes = b.lds_bytes;
+        b.v[0].name = b.wide ? "wide NT" : "LDS KB";
         if (!tune) continue;
         int maxd = 0, maxp = 0;
         for (int t = 0; t < n_tasks; ++t)
@@ -502,26 +537,42 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
                 maxd = std::max(maxd, tasks[t].dim);
                 maxp = std::max(maxp, tasks[t].pop);
             }
+        // the wide kernel with Q in LDS for row geometry nt, when its LDS fits
+        auto add_wideq = [&](int nt) {
+            int (*ql)(int) = nullptr;
+            launch_fn qf = pick_wideq(nt, ga, &ql);
+            if (!qf || 4 * ql(maxp) > kMaxLdsBytes) return;
+            b.v[2].wide = true;
+            b.v[2].geo = nt;
+            b.v[2].fn = qf;
+            b.v[2].lds_bytes = 4 * ql(maxp);
+            b.v[2].name = "wide Q-in-LDS NT";
+        };
         if (b.wide) {
             // wide bucket: the 512-thread build when some population spans more than four tiles
             if (maxp > 128) {
                 b.v[1] = b.v[0];
                 b.v[1].fn = pick_wide8(b.geo, ga);
+                b.v[1].name = "wide x512 NT";
             }
+            add_wideq(b.geo);
             continue;
         }
-        // an LDS-kernel bucket with two or more workgroups per CU keeps its kernel: the wide
-        // kernel (rows in L2) only competes where LDS allows a single workgroup
-        if (2 * b.lds_bytes <= kMaxLdsBytes) continue;
+        // an LDS-kernel bucket: the wide kernel with Q in LDS always competes; the wide
+        // kernel with Q in L2 only where LDS allows a single workgroup of the LDS kernel
         const int nt = pick_nt(maxd);
+        if (nt <= 0) continue;
+        add_wideq(nt);
         int (*wl)(int) = nullptr;
-        launch_fn wf = nt > 0 ? pick_wide(nt, ga, &wl) : nullptr;
-        if (!wf) continue;
-        b.v[1].wide = true;
-        b.v[1].geo = nt;
-        b.v[1].fn = wf;
-        b.v[1].lds_bytes = 4 * wl(maxp);
-        alt_nt[k] = nt;
+        launch_fn wf = 2 * b.lds_bytes > kMaxLdsBytes ? pick_wide(nt, ga, &wl) : nullptr;
+        if (wf) {
+            b.v[1].wide = true;
+            b.v[1].geo = nt;
+            b.v[1].fn = wf;
+            b.v[1].lds_bytes = 4 * wl(maxp);
+            b.v[1].name = "wide NT";
+        }
+        if (b.v[1].fn || b.v[2].fn) alt_nt[k] = nt;
     }
     for (int t = 0; t < n_tasks; ++t)
         if (alt_nt[bucket_of[t]] > 0) wide_nt[t] = alt_nt[bucket_of[t]];
@@ -721,8 +772,11 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, std::string("init: ") + hipGetErrorString(e)); }
     if (tune && prm->tune == 0 && (rc = calibrate(h, nX, nF)) != EMTF_OK) { free_handle(h); return rc; }
-    if (tune && prm->tune == 2)
-        for (Bucket& b : h->buckets) b.use = b.v[1].fn ? 1 : 0;
+    if (tune && prm->tune >= 2)
+        for (Bucket& b : h->buckets) {
+            const int v = prm->tune - 1;
+            b.use = b.v[v].fn ? v : 0;
+        }
     *out = h;
     return EMTF_OK;
 }

@@ -1273,7 +1273,7 @@ __device__ __forceinline__ float wide_eval_row(const float* row_h, const Geo& g,
     return objective<NT, FUNC, true>(acc, g.D, L.tab, h);
 }
 
-template <int NT, int FUNC, int ALGO, int WT>
+template <int NT, int FUNC, int ALGO, int WT, bool QL>
 __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
                                                 float* T, const Rng& rng, int gen, int tid, int nsb) {
     using S = WShape<NT>;
@@ -1299,16 +1299,20 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
             const bool odd = (il & 1) != 0;
             float4 na = lds4(xa), nb = lds4(xb);
             float4 nq[NT];
-            load_q<NT>(nq, qrow, 0);
+            if constexpr (!QL) load_q<NT>(nq, qrow, 0);
             for (int sb = 0; sb < nsb; ++sb) {
                 const float4 va = na, vb = nb;
                 float4 q[NT];
+                if constexpr (QL) {
+                    load_q<NT>(q, qrow, sb);
+                } else {
 #pragma unroll
-                for (int jt = 0; jt < NT; ++jt) q[jt] = nq[jt];
+                    for (int jt = 0; jt < NT; ++jt) q[jt] = nq[jt];
+                }
                 if (sb + 1 < nsb) {
                     na = lds4(xa + 4 * (sb + 1));
                     nb = lds4(xb + 4 * (sb + 1));
-                    load_q<NT>(nq, qrow, sb + 1);
+                    if constexpr (!QL) load_q<NT>(nq, qrow, sb + 1);
                 }
                 float t[4];
                 ga_block(t, va, vb, rng, gen, ie, sb, h, odd, g.D, c0);
@@ -1327,19 +1331,23 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
             U4 cb{0u, 0u, 0u, 0u};
             float4 nb = lds4(xb), n2 = lds4(x2), n3 = lds4(x3), ni = lds4(xi);
             float4 nq[NT];
-            load_q<NT>(nq, qrow, 0);
+            if constexpr (!QL) load_q<NT>(nq, qrow, 0);
             for (int sb = 0; sb < nsb; ++sb) {
                 const float4 vb = nb, v2 = n2, v3 = n3, vi = ni;
                 float4 q[NT];
+                if constexpr (QL) {
+                    load_q<NT>(q, qrow, sb);
+                } else {
 #pragma unroll
-                for (int jt = 0; jt < NT; ++jt) q[jt] = nq[jt];
+                    for (int jt = 0; jt < NT; ++jt) q[jt] = nq[jt];
+                }
                 if (sb + 1 < nsb) {
                     const int o = 4 * (sb + 1);
                     nb = lds4(xb + o);
                     n2 = lds4(x2 + o);
                     n3 = lds4(x3 + o);
                     ni = lds4(xi + o);
-                    load_q<NT>(nq, qrow, sb + 1);
+                    if constexpr (!QL) load_q<NT>(nq, qrow, sb + 1);
                 }
                 if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
                 float t[4];
@@ -1439,7 +1447,9 @@ __device__ __forceinline__ void wide_transfer_in(const EpochArgs& a, const TaskD
     __syncthreads();
 }
 
-template <int NT, int FUNC, int ALGO, int WT>
+// QL: the task's Q image is staged in LDS (the workgroup's first 32 NT x RS floats) and
+// the MFMA A operands come from there instead of L2.
+template <int NT, int FUNC, int ALGO, int WT, bool QL>
 __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
     using S = WShape<NT>;
     constexpr int RS = S::RS, HS = S::HS;
@@ -1457,8 +1467,17 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     Lds L;
     L.X = a.scratch + td.scr_off;
     float* T = L.X + (int64_t)g.popP * RS;
-    L.Q = const_cast<float*>(a.Qw + td.qw_off);
-    L.o = smem;
+    float* base = smem;
+    if constexpr (QL) {
+        L.Q = smem;
+        base = smem + 32 * NT * RS;
+        const float* qg = a.Qw + td.qw_off;
+        for (int e = tid; e < 32 * NT * RS / 4; e += WT)
+            reinterpret_cast<float4*>(smem)[e] = reinterpret_cast<const float4*>(qg)[e];
+    } else {
+        L.Q = const_cast<float*>(a.Qw + td.qw_off);
+    }
+    L.o = base;
     L.tab = L.o + RS;
     L.fit = L.tab + 32 * NT;
     L.ftmp = L.fit + g.popP;
@@ -1467,7 +1486,7 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     const Rng rng{a.p.seed_lo, a.p.seed_hi, (uint32_t)task};
     const int lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
 
-    for (int e = tid; e < RS + 32 * NT; e += WT) smem[e] = 0.0f;
+    for (int e = tid; e < RS + 32 * NT; e += WT) base[e] = 0.0f;
     __syncthreads();
     for (int d = tid; d < g.D; d += WT) {
         L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
@@ -1511,7 +1530,7 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
         if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
             island_exchange<4 * NT, WT>(a, g, L, T, rng, gen, tid);
-        wide_generation<NT, FUNC, ALGO, WT>(a, td, g, L, T, rng, gen, tid, nsb);
+        wide_generation<NT, FUNC, ALGO, WT, QL>(a, td, g, L, T, rng, gen, tid, nsb);
         if (islands && a.p.migrate_interval > 0 && a.p.migrate_count > 0 && gen % a.p.migrate_interval == 0)
             island_migration<4 * NT, WT>(a, g, L, tid);
     }
@@ -1545,20 +1564,20 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
 #define EMTF_WIDE_OCCUPANCY
 #endif
 
-template <int NT, int ALGO, int WT>
+template <int NT, int ALGO, int WT, bool QL>
 __global__ __launch_bounds__(WT) EMTF_WIDE_OCCUPANCY void wide_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
     extern __shared__ float smem[];
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
     switch (td.func) {
-    case F_SPHERE: wide_body<NT, F_SPHERE, ALGO, WT>(a, smem, task, td); break;
-    case F_ROSENBROCK: wide_body<NT, F_ROSENBROCK, ALGO, WT>(a, smem, task, td); break;
-    case F_ACKLEY: wide_body<NT, F_ACKLEY, ALGO, WT>(a, smem, task, td); break;
-    case F_RASTRIGIN: wide_body<NT, F_RASTRIGIN, ALGO, WT>(a, smem, task, td); break;
-    case F_GRIEWANK: wide_body<NT, F_GRIEWANK, ALGO, WT>(a, smem, task, td); break;
-    case F_WEIERSTRASS: wide_body<NT, F_WEIERSTRASS, ALGO, WT>(a, smem, task, td); break;
-    case F_SCHWEFEL: wide_body<NT, F_SCHWEFEL, ALGO, WT>(a, smem, task, td); break;
-    case F_ELLIPTIC: wide_body<NT, F_ELLIPTIC, ALGO, WT>(a, smem, task, td); break;
+    case F_SPHERE: wide_body<NT, F_SPHERE, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_ROSENBROCK: wide_body<NT, F_ROSENBROCK, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_ACKLEY: wide_body<NT, F_ACKLEY, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_RASTRIGIN: wide_body<NT, F_RASTRIGIN, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_GRIEWANK: wide_body<NT, F_GRIEWANK, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_WEIERSTRASS: wide_body<NT, F_WEIERSTRASS, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_SCHWEFEL: wide_body<NT, F_SCHWEFEL, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_ELLIPTIC: wide_body<NT, F_ELLIPTIC, ALGO, WT, QL>(a, smem, task, td); break;
     default: break;
     }
 }
@@ -1581,17 +1600,40 @@ extern "C" int EMTF_WNAME(emtf_wide_lds_floats_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(i
 
 extern "C" int EMTF_WNAME(emtf_launch_wide_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
                                                                           int n, int lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 256>), dim3(n), dim3(256), lds_bytes, stream,
+    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 256, false>), dim3(n), dim3(256), lds_bytes, stream,
                        *args, order);
     return (int)hipGetLastError();
 }
 
 extern "C" int EMTF_WNAME(emtf_launch_wide8_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
                                                                            int n, int lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512>), dim3(n), dim3(512), lds_bytes, stream,
+    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512, false>), dim3(n), dim3(512), lds_bytes, stream,
                        *args, order);
     return (int)hipGetLastError();
 }
+
+// 512 threads with the Q image in LDS (lds_bytes includes it): built where it fits
+#if EMTF_WIDE_NT <= 6
+extern "C" int EMTF_WNAME(emtf_launch_wideq_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
+                                                                           int n, int lds_bytes, hipStream_t stream) {
+    static bool lds_ok = false;
+    auto k = &emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512, true>;
+    if (!lds_ok) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return (int)e;
+        lds_ok = true;
+    }
+    hipLaunchKernelGGL(k, dim3(n), dim3(512), lds_bytes, stream, *args, order);
+    return (int)hipGetLastError();
+}
+
+extern "C" int EMTF_WNAME(emtf_wideq_lds_floats_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(int pop) {
+    const int popP = (pop + 31) & ~31;
+    using S = emtf::WShape<EMTF_WIDE_NT>;
+    return 32 * EMTF_WIDE_NT * S::RS + S::RS + 32 * EMTF_WIDE_NT + 4 * popP;
+}
+#endif
 #else
 #if !defined(EMTF_KB) || !defined(EMTF_ALGO)
 #error "compile island.hip with -DEMTF_KB=<gene blocks per half> -DEMTF_ALGO=<0|2>"

@@ -170,10 +170,11 @@ def test_persistent_many_epochs_per_call(oracle_mod):
     _compare(per, orc, "1030 one-generation epochs")
 
 
-@pytest.mark.parametrize("tune", [0, 1, 2])
+@pytest.mark.parametrize("tune", [0, 1, 2, 3])
 def test_autotune_is_bit_exact(oracle_mod, tune):
-    """The autotuner times the LDS and the wide kernel of each bucket at create (and
-    restores the state), then keeps one; every epoch must agree with the oracle either way."""
+    """The autotuner times the LDS kernel, the wide kernel and the wide kernel with Q in LDS
+    of each bucket at create (and restores the state), then keeps one; every epoch must
+    agree with the oracle whichever runs (tune 2 and 3 force the alternatives)."""
     tasks = tg.uneven_tasks(40, seed=8, dmax=220, pmax=256)
     params = dict(seed=3, transfer_interval=2, transfer_count=2, n_islands=2, migrate_interval=3, migrate_count=2)
     gpu = emtf.Engine(tasks, tune=tune, **params)
@@ -194,3 +195,15 @@ def test_wide_512_threads_ga(oracle_mod):
     gpu.run(7)
     orc.run(7)
     _compare(gpu, orc, "wide x512 GA")
+
+
+@pytest.mark.parametrize("algo", [0, 2])
+def test_wide_q_in_lds(oracle_mod, algo):
+    """The wide kernel with the task's Q image staged in LDS (tune=3), NT 5 and 6 rows."""
+    tasks = tg.benchmark_tasks(4, 170, 224, rotate=True, seed=17) + tg.benchmark_tasks(4, 150, 96, rotate=True, seed=18)
+    params = dict(seed=6, algo=algo, n_islands=4, exchange_interval=2, migrate_interval=3,
+                  migrate_count=3, transfer_interval=3, transfer_count=4)
+    gpu, orc = _pair(tasks, oracle_mod, tune=3, **params)
+    gpu.run(7)
+    orc.run(7)
+    _compare(gpu, orc, f"wide Q in LDS algo {algo}")
