@@ -1558,14 +1558,12 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
 // formation and objective epilogue (one wave per SIMD leaves the MFMA pipe idle
 // through both).  Workgroups of 256 threads (two per CU) or, for populations of
 // more than four 32-row tiles, 512 threads (one per CU, half the tiles per wave).
-#if defined(EMTF_WIDE_NT) && EMTF_WIDE_NT <= 6
-#define EMTF_WIDE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2, 2)))
-#else
-#define EMTF_WIDE_OCCUPANCY
-#endif
+// WPE: the waves per SIMD the register allocation targets (1: no target)
+constexpr int kWideWpe = EMTF_WIDE_NT <= 6 ? 2 : 1;
 
-template <int NT, int ALGO, int WT, bool QL>
-__global__ __launch_bounds__(WT) EMTF_WIDE_OCCUPANCY void wide_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
+template <int NT, int ALGO, int WT, bool QL, int WPE>
+__global__ __launch_bounds__(WT) __attribute__((amdgpu_waves_per_eu(WPE))) void wide_epoch_kernel(
+    EpochArgs a, const int* __restrict__ order) {
     extern __shared__ float smem[];
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
@@ -1600,14 +1598,14 @@ extern "C" int EMTF_WNAME(emtf_wide_lds_floats_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(i
 
 extern "C" int EMTF_WNAME(emtf_launch_wide_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
                                                                           int n, int lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 256, false>), dim3(n), dim3(256), lds_bytes, stream,
+    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 256, false, emtf::kWideWpe>), dim3(n), dim3(256), lds_bytes, stream,
                        *args, order);
     return (int)hipGetLastError();
 }
 
 extern "C" int EMTF_WNAME(emtf_launch_wide8_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
                                                                            int n, int lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512, false>), dim3(n), dim3(512), lds_bytes, stream,
+    hipLaunchKernelGGL((emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512, false, emtf::kWideWpe>), dim3(n), dim3(512), lds_bytes, stream,
                        *args, order);
     return (int)hipGetLastError();
 }
@@ -1617,7 +1615,7 @@ extern "C" int EMTF_WNAME(emtf_launch_wide8_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(cons
 extern "C" int EMTF_WNAME(emtf_launch_wideq_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(const emtf::EpochArgs* args, const int* order,
                                                                            int n, int lds_bytes, hipStream_t stream) {
     static bool lds_ok = false;
-    auto k = &emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512, true>;
+    auto k = &emtf::wide_epoch_kernel<EMTF_WIDE_NT, EMTF_ALGO, 512, true, emtf::kWideWpe>;
     if (!lds_ok) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
