@@ -115,7 +115,27 @@ int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_strid
  * messageTransfer with BESTNUM_PERTASK = k (SG-EMTF.cu:60, :462-486, :1601-1615). */
 int emtf_get_elites(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t x_stride);
 
+/* Convergence trace recorded on the device (the reference's per-task record, SG-EMTF_DE.cu:
+ * 1793-1811: best at generation g0 and after every `interval` generations, then the worst
+ * of the population at g0).  Records point 0 now (g0 = the handle's generation) and then,
+ * during emtf_run, the best of every task at g0 + k interval, up to max_points points; each
+ * point is one small kernel after the epoch that ends there, no host synchronisation.
+ * interval 0 stops recording (the points stay readable); a new start replaces the trace. */
+int emtf_trace_start(emtf_handle* h, int32_t interval, int32_t max_points, void* stream);
+
+/* Points recorded so far: *n_points, best[n_points][n_tasks] and initial_worst[n_tasks]
+ * (either may be NULL); synchronises.  Best / worst ignore NaN fitness. */
+int emtf_trace(emtf_handle* h, int32_t* n_points, float* best, float* initial_worst);
+
 int emtf_destroy(emtf_handle* h);
+
+/* The engine's random draws (device pointers, asynchronous on `stream`): out[i][0..3] =
+ * the Philox4x32-10 block that the engine draws at address addrs[i] = {task, generation,
+ * purpose, index, sub, block} under `seed` (key = seed halves; counter = {block,
+ * purpose<<24 | (sub & 0xFF)<<16 | (index & 0xFFFF), generation, task}).  out is 16-byte
+ * aligned.  Lets a checker consume the exact streams of a run.  Replaces the per-thread
+ * cuRAND XORWOW states (SG-EMTF.cu:163-179), which had no export. */
+int emtf_eval_draws(uint64_t seed, int32_t n, const uint32_t* addrs, uint32_t* out, void* stream);
 
 /* ---- Objective and adjoint (device pointers, asynchronous on `stream`) -------
  * f[i] = f(Q (x_i - o)) for n points x[n][dim], bit for bit the engine's evaluator

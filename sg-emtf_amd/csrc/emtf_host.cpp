@@ -56,6 +56,9 @@ EMTF_DECL_QPAIR(2) EMTF_DECL_QPAIR(4) EMTF_DECL_QPAIR(5) EMTF_DECL_QPAIR(6)
 EMTF_DECL_OBJ(2) EMTF_DECL_OBJ(4) EMTF_DECL_OBJ(7) EMTF_DECL_OBJ(8) EMTF_DECL_OBJ(12) EMTF_DECL_OBJ(13) EMTF_DECL_OBJ(16)
 
 typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
+extern "C" int emtf_launch_trace(const emtf::TaskDev* tasks, int n_tasks, const float* fit, float* out_best,
+                                 float* out_worst, hipStream_t stream);
+extern "C" int emtf_launch_draws(uint64_t seed, int n, const uint32_t* addr, uint32_t* out, hipStream_t stream);
 extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const float* X, const float* fit, int k,
                                   int stride, float* out_x, float* out_f, hipStream_t stream);
 
@@ -214,6 +217,10 @@ struct emtf_handle {
     std::vector<hipStream_t> side;  // extra streams for buckets 1.. (non-blocking, default priority)
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join;
+    // convergence trace: best of every task at trace_g0 + k trace_interval, k < trace_max
+    int trace_interval = 0, trace_max = 0, trace_points = 0, trace_g0 = 0;
+    float* d_trace = nullptr;        // [trace_max][T]
+    float* d_trace_worst = nullptr;  // [T], the worst fitness at trace_g0
 };
 
 static void free_handle(emtf_handle* h) {
@@ -228,6 +235,8 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_scratch);
     (void)hipFree(h->d_qw);
     (void)hipFree(h->d_order);
+    (void)hipFree(h->d_trace);
+    (void)hipFree(h->d_trace_worst);
     for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
 
     for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
@@ -801,10 +810,60 @@ int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
             const int nb = (h->gen / PI + 1) * PI;
             if (nb < g1) g1 = nb;
         }
+        const int TR = h->trace_points < h->trace_max ? h->trace_interval : 0;
+        if (TR > 0) {
+            const int nb = h->trace_g0 + ((h->gen - h->trace_g0) / TR + 1) * TR;
+            if (nb < g1) g1 = nb;
+        }
         const int rc = launch(h, h->gen, g1, 0, s);
         if (rc != EMTF_OK) return rc;
         h->gen = g1;
+        if (TR > 0 && (g1 - h->trace_g0) % TR == 0) {
+            const int e = emtf_launch_trace(h->d_tasks, h->T, h->d_fit, h->d_trace + (int64_t)h->trace_points * h->T,
+                                            nullptr, s);
+            if (e != 0) return fail(EMTF_E_DEVICE, std::string("trace: ") + hipGetErrorString((hipError_t)e));
+            ++h->trace_points;
+        }
     }
+    return EMTF_OK;
+}
+
+int emtf_trace_start(emtf_handle* h, int32_t interval, int32_t max_points, void* stream) {
+    g_err.clear();
+    if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
+    if (interval < 0) return fail(EMTF_E_INVALID, "interval < 0");
+    if (interval == 0) {
+        h->trace_interval = 0;
+        return EMTF_OK;
+    }
+    if (max_points < 1) return fail(EMTF_E_INVALID, "max_points < 1");
+    h->stream = (hipStream_t)stream;
+    HIP_OR_FAIL(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->d_trace);
+    (void)hipFree(h->d_trace_worst);
+    h->d_trace = h->d_trace_worst = nullptr;
+    h->trace_interval = h->trace_max = h->trace_points = 0;
+    HIP_OR_FAIL(hipMalloc((void**)&h->d_trace, sizeof(float) * (size_t)max_points * h->T));
+    HIP_OR_FAIL(hipMalloc((void**)&h->d_trace_worst, sizeof(float) * (size_t)h->T));
+    const int e = emtf_launch_trace(h->d_tasks, h->T, h->d_fit, h->d_trace, h->d_trace_worst, h->stream);
+    if (e != 0) return fail(EMTF_E_DEVICE, std::string("trace: ") + hipGetErrorString((hipError_t)e));
+    h->trace_interval = interval;
+    h->trace_max = max_points;
+    h->trace_points = 1;
+    h->trace_g0 = h->gen;
+    return EMTF_OK;
+}
+
+int emtf_trace(emtf_handle* h, int32_t* n_points, float* best, float* initial_worst) {
+    g_err.clear();
+    if (!h || !n_points) return fail(EMTF_E_INVALID, "NULL argument");
+    HIP_OR_FAIL(hipStreamSynchronize(h->stream));
+    *n_points = h->trace_points;
+    if (h->trace_points == 0) return EMTF_OK;
+    if (best)
+        HIP_OR_FAIL(hipMemcpy(best, h->d_trace, sizeof(float) * (size_t)h->trace_points * h->T, hipMemcpyDeviceToHost));
+    if (initial_worst)
+        HIP_OR_FAIL(hipMemcpy(initial_worst, h->d_trace_worst, sizeof(float) * (size_t)h->T, hipMemcpyDeviceToHost));
     return EMTF_OK;
 }
 
@@ -907,6 +966,17 @@ int emtf_objective_fwd(int32_t func, int32_t dim, int32_t n, const float* x, con
 int emtf_objective_adjoint(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
                            const float* gf, float* gx, void* stream) {
     return objective_call(func, dim, n, x, rot, shift, gf, nullptr, gx, stream, 1);
+}
+
+int emtf_eval_draws(uint64_t seed, int32_t n, const uint32_t* addrs, uint32_t* out, void* stream) {
+    g_err.clear();
+    if (n < 0) return fail(EMTF_E_INVALID, "n < 0");
+    if (n == 0) return EMTF_OK;
+    if (!addrs || !out) return fail(EMTF_E_INVALID, "addrs or out is NULL");
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(EMTF_E_INVALID, "out is not 16-byte aligned");
+    const int rc = emtf_launch_draws(seed, n, addrs, out, (hipStream_t)stream);
+    if (rc != 0) return fail(EMTF_E_DEVICE, std::string("draws: ") + hipGetErrorString((hipError_t)rc));
+    return EMTF_OK;
 }
 
 int emtf_get_elites(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t x_stride) {

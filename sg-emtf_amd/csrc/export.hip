@@ -1,4 +1,6 @@
-// export.hip -- elite export: the k best individuals of every task, ranked on the device.
+// export.hip -- elite export (the k best individuals of every task, ranked on the device),
+// the convergence trace (best / worst fitness of every task) and draw export (the engine's
+// Philox blocks at given addresses).
 //
 // Replaces the reference's popSort + messageTransfer pair (proposed model/SG-EMTF.cu:462-486,
 // :1601-1615), which bitonic-sorts 512 (fitness, pointer) pairs per task in a separate
@@ -8,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "emtf_types.h"
+#include "emtf_math.h"
 
 namespace emtf {
 
@@ -41,7 +44,66 @@ __global__ __launch_bounds__(256) void elite_kernel(const TaskDev* __restrict__ 
     for (int r = tid; r < kk; r += blockDim.x) out_f[(int64_t)t * k + r] = fit[td.fit_off + ord[r]];
 }
 
+// One Philox4x32-10 block per address {task, generation, purpose, index, sub, block},
+// through the engine's own Rng (the call the epoch kernels make).
+__global__ __launch_bounds__(256) void draws_kernel(uint32_t seed_lo, uint32_t seed_hi, int n,
+                                                    const uint32_t* __restrict__ addr, uint32_t* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* a = addr + 6 * (int64_t)i;
+    const Rng rng{seed_lo, seed_hi, a[0]};
+    const U4 b = rng.block(a[1], a[2], a[3], a[4], a[5]);
+    uint4 v;
+    v.x = b.x;
+    v.y = b.y;
+    v.z = b.z;
+    v.w = b.w;
+    reinterpret_cast<uint4*>(out)[i] = v;
+}
+
+// Best (and, with out_worst, worst) fitness of every task, NaN ignored (NaN when every
+// individual is NaN); one workgroup per task.  min / max are exact in any order.
+__global__ __launch_bounds__(256) void trace_kernel(const TaskDev* __restrict__ tasks, const float* __restrict__ fit,
+                                                    float* __restrict__ out_best, float* __restrict__ out_worst) {
+    __shared__ float smin[256], smax[256];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const TaskDev td = tasks[t];
+    const float nan = __builtin_nanf("");
+    float mn = nan, mx = nan;
+    for (int i = tid; i < td.pop; i += 256) {
+        const float f = fit[td.fit_off + i];
+        mn = __builtin_fminf(mn, f);
+        mx = __builtin_fmaxf(mx, f);
+    }
+    smin[tid] = mn;
+    smax[tid] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            smin[tid] = __builtin_fminf(smin[tid], smin[tid + w]);
+            smax[tid] = __builtin_fmaxf(smax[tid], smax[tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out_best[t] = smin[0];
+        if (out_worst) out_worst[t] = smax[0];
+    }
+}
+
 }  // namespace emtf
+
+extern "C" int emtf_launch_trace(const emtf::TaskDev* tasks, int n_tasks, const float* fit, float* out_best,
+                                 float* out_worst, hipStream_t stream) {
+    hipLaunchKernelGGL(emtf::trace_kernel, dim3(n_tasks), dim3(256), 0, stream, tasks, fit, out_best, out_worst);
+    return (int)hipGetLastError();
+}
+
+extern "C" int emtf_launch_draws(uint64_t seed, int n, const uint32_t* addr, uint32_t* out, hipStream_t stream) {
+    hipLaunchKernelGGL(emtf::draws_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), n, addr, out);
+    return (int)hipGetLastError();
+}
 
 extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const float* X, const float* fit, int k,
                                   int stride, float* out_x, float* out_f, hipStream_t stream) {

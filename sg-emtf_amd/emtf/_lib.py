@@ -56,10 +56,13 @@ def lib():
         L.emtf_set_population.argtypes = [vp, ctypes.c_int32, f32p, f32p]
         L.emtf_get_best.argtypes = [vp, f32p, f32p, ctypes.c_int32]
         L.emtf_get_elites.argtypes = [vp, ctypes.c_int32, f32p, f32p, ctypes.c_int32]
+        L.emtf_trace_start.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+        L.emtf_trace.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), f32p, f32p]
         L.emtf_destroy.argtypes = [vp]
         L.emtf_last_error.restype = ctypes.c_char_p
         for n in ["emtf_create", "emtf_run", "emtf_sync", "emtf_generation", "emtf_get_population",
-                  "emtf_set_population", "emtf_get_best", "emtf_get_elites", "emtf_destroy"]:
+                  "emtf_set_population", "emtf_get_best", "emtf_get_elites", "emtf_trace_start", "emtf_trace",
+                  "emtf_destroy"]:
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib

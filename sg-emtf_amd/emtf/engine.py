@@ -96,6 +96,21 @@ class Engine:
         _lib.check(_lib.lib().emtf_get_elites(self._h, int(k), _fp(f), _fp(x), self.max_dim))
         return f, x
 
+    def trace_start(self, interval, max_points, stream=None):
+        """Record every task's best on the device now and every `interval` generations
+        (at most max_points points); interval 0 stops recording."""
+        _lib.check(_lib.lib().emtf_trace_start(self._h, int(interval), int(max_points), ctypes.c_void_p(stream or 0)))
+
+    def trace(self):
+        """The recorded trace as [n_tasks, points + 1] float32 rows: the best at each point,
+        then the worst fitness at the first point (ConvergenceRecorder.rows' layout)."""
+        n = ctypes.c_int32()
+        _lib.check(_lib.lib().emtf_trace(self._h, ctypes.byref(n), None, None))
+        best = np.zeros((max(n.value, 1), self.n_tasks), np.float32)
+        worst = np.zeros(self.n_tasks, np.float32)
+        _lib.check(_lib.lib().emtf_trace(self._h, ctypes.byref(n), _fp(best), _fp(worst)))
+        return np.concatenate([best[: n.value].T, worst[:, None]], 1)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().emtf_destroy(self._h)

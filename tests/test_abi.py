@@ -141,3 +141,37 @@ def test_c_example_runs(tmp_path, oracle_mod):
     best = [np.float32(np.min(f)) for f in fo]
     # "%.9g" round-trips binary32
     assert [np.float32(v) for v in out[1:3]] == best
+
+
+def test_trace_argument_checks():
+    from emtf import _lib
+    L = _lib.lib()
+    n = ctypes.c_int32()
+    assert L.emtf_trace_start(None, 5, 4, None) == -1
+    assert L.emtf_trace(None, ctypes.byref(n), None, None) == -1
+
+
+def test_eval_draws_argument_checks():
+    from emtf import _lib
+    from emtf.draws import _bind
+    L = _bind()
+    assert L.emtf_eval_draws(1, 0, None, None, None) == 0
+    assert L.emtf_eval_draws(1, -1, None, None, None) == -1
+    assert L.emtf_eval_draws(1, 4, None, None, None) == -1 and "NULL" in _lib.lib().emtf_last_error().decode()
+
+
+@pytest.mark.gpu
+def test_eval_draws_match_oracle(oracle_mod):
+    """Device Philox blocks at random addresses equal the oracle's (pinned by rocRAND and
+    Random123 known answers), including index / sub values past their 16 / 8 counter bits."""
+    from emtf.draws import eval_draws
+    rng = np.random.default_rng(5)
+    n = 1000
+    addrs = np.stack([rng.integers(0, 5000, n), rng.integers(0, 2 ** 32, n, dtype=np.int64), rng.integers(1, 18, n),
+                      rng.integers(0, 70000, n), rng.integers(0, 300, n), rng.integers(0, 2 ** 32, n, dtype=np.int64)], 1)
+    seed = 0xFEDCBA9876543210
+    got = eval_draws(seed, addrs).numpy()
+    for i in range(n):
+        t, g, p, idx, sub, blk = (int(v) for v in addrs[i])
+        want = oracle_mod.draw_block(seed, t, g, p, idx & 0xFFFF, sub & 0xFF, blk)
+        assert list(got[i]) == want, (i, addrs[i])

@@ -118,3 +118,35 @@ def test_elite_export_matches_population_order():
         assert not x[t, :, D:].any()
     fb, xb = eng.best()
     assert np.array_equal(fb.view(np.uint32), f[:, 0].view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_device_trace_matches_oracle(oracle_mod):
+    """emtf_trace_start / emtf_trace record on the device what a host loop over the oracle
+    records (best every 7 generations, initial worst last), with runs whose lengths do not
+    line up with the interval; the traced run's populations stay bit-exact."""
+    import emtf
+    tasks = tg.uneven_tasks(10, seed=4)
+    params = dict(seed=8, transfer_interval=5, transfer_count=2)
+    eng = emtf.Engine(tasks, **params)
+    orc = oracle_mod.Engine(tasks, **params)
+    eng.run(3)
+    orc.run(3)
+    eng.trace_start(7, 6)
+    want_best = [orc.best()[0]]
+    worst = np.array([np.nanmax(orc.population(t)[1]) for t in range(len(tasks))], np.float32)
+    for step in (5, 13, 1, 30):
+        eng.run(step)
+    for _ in range(6 - 1):
+        orc.run(7)
+        want_best.append(orc.best()[0])
+    orc.run(3 + 49 - orc.generation)
+    rows = eng.trace()
+    assert rows.shape == (len(tasks), 7)
+    want = np.concatenate([np.stack(want_best, 1), worst[:, None]], 1)
+    assert np.array_equal(rows.view(np.uint32), want.view(np.uint32))
+    for t in range(len(tasks)):
+        assert np.array_equal(eng.population(t)[0].view(np.uint32), orc.population(t)[0].view(np.uint32))
+    eng.trace_start(0, 0)
+    eng.run(10)
+    assert eng.trace().shape == (len(tasks), 7)
