@@ -37,10 +37,12 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-gens", type=int, default=100, help="generations of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
-    ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "cvrp", "polish"],
+    ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "cvrp", "polish", "online"],
                     help="plain: the rotated continuous workload (default); uneven: 4096 tasks with "
                          "D~U{10..200}, pop~U{32..256}; cvrp: the CVRP GA workload; "
-                         "polish: time to mean error 1e-6, adjoint polish vs plain DE at D=100")
+                         "polish: time to mean error 1e-6, adjoint polish vs plain DE at D=100; "
+                         "online: 10000 plain tasks admitted in 100 batches at U[0,100) ms gaps")
+    ap.add_argument("--batches", type=int, default=100, help="online workload: arrival batches")
     ap.add_argument("--seeds", type=int, default=10, help="polish workload: seeds 0..N-1")
     ap.add_argument("--no-tune", action="store_true", help="switch the kernel-variant / stream autotuner off")
     ap.add_argument("--gens", type=int, default=3000, help="polish workload: generations per run")
@@ -224,6 +226,45 @@ def bench_polish(a):
     print(json.dumps(out))
 
 
+def bench_online(a):
+    """Online admission (the reference's driver, SG-EMTF.cu:1842-1958): T tasks of the plain
+    configuration split into batches at random cut points, batch i arriving U[0, 100) ms after
+    batch i-1; each batch is created when it arrives and runs 2000 generations on its own
+    stream while the earlier ones still run.  Reports the mean response time (arrival -> its
+    generations queued, creation included) and solve time (arrival -> finished) per task."""
+    import numpy as np
+    import emtf
+    from emtf import online
+    from emtf import tasks as tg
+    T = a.tasks if a.tasks != 1024 else 10000
+    gens = a.interval * a.steps
+    tasks = tg.benchmark_tasks(T, a.dim, a.pop, rotate=True, seed=a.seed)
+    cuts, arr = online.arrival_schedule(T, a.batches, max_gap_ms=100.0, seed=a.seed)
+    params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval, transfer_count=a.elites)
+    # one untimed batch loads the code objects
+    warm = online.OnlineRun(tasks[:8], [0, 8], [0.0], a.interval, seed=a.seed, **params)
+    warm.run()
+    warm.close()
+    run = online.OnlineRun(tasks, cuts, arr, gens, seed=a.seed, **params)
+    resp, solve = run.run()
+    best = run.best()
+    run.close()
+    span = run.makespan_ms
+    out = {"metric": "online admission: mean solve time per task (arrival -> finished)", "value": float(solve.mean()),
+           "unit": "ms", "n_gpus": 1, "steps": a.steps, "warmup": 1, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"online: {T} rotated tasks D={a.dim} pop={a.pop} in {a.batches} batches, "
+                                  f"gaps U[0,100) ms, {gens} generations each, DE/rand/1/bin F=0.5 CR=0.9, "
+                                  f"{a.elites} elites every {a.interval} gens inside a batch",
+                      "tasks": T, "batches": a.batches, "generations": gens},
+           "mean_response_ms": float(resp.mean()), "max_response_ms": float(resp.max()),
+           "mean_solve_ms": float(solve.mean()), "max_solve_ms": float(solve.max()),
+           "last_arrival_ms": float(arr[-1]), "makespan_ms": span,
+           "task_generations_per_s_over_makespan": T * gens / (span / 1e3),
+           "mean_best_fitness": float(np.mean(best))}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -234,6 +275,9 @@ def main():
         return
     if a.workload == "polish":
         bench_polish(a)
+        return
+    if a.workload == "online":
+        bench_online(a)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine shards tasks across "

@@ -13,6 +13,10 @@ def hip():
         h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
         h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        h.hipEventQuery.argtypes = [ctypes.c_void_p]
+        h.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        h.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+        h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
         h.hipGetErrorString.restype = ctypes.c_char_p
         _hip = h
     return _hip
@@ -38,6 +42,14 @@ class Event:
     def synchronize(self):
         _ok(hip().hipEventSynchronize(self.e))
 
+    def query(self):
+        """True once the work recorded before the event has finished."""
+        rc = hip().hipEventQuery(self.e)
+        if rc == 600:  # hipErrorNotReady
+            return False
+        _ok(rc)
+        return True
+
     def elapsed_ms(self, end):
         ms = ctypes.c_float()
         _ok(hip().hipEventElapsedTime(ctypes.byref(ms), self.e, end.e))
@@ -46,5 +58,29 @@ class Event:
     def __del__(self):
         try:
             hip().hipEventDestroy(self.e)
+        except Exception:
+            pass
+
+
+HIP_STREAM_NON_BLOCKING = 0x1
+
+
+class Stream:
+    """A non-blocking stream at the default priority (no implicit ordering with the null stream)."""
+
+    def __init__(self):
+        self.s = ctypes.c_void_p()
+        _ok(hip().hipStreamCreateWithFlags(ctypes.byref(self.s), HIP_STREAM_NON_BLOCKING))
+
+    @property
+    def handle(self):
+        return self.s.value
+
+    def synchronize(self):
+        _ok(hip().hipStreamSynchronize(self.s))
+
+    def __del__(self):
+        try:
+            hip().hipStreamDestroy(self.s)
         except Exception:
             pass
