@@ -197,7 +197,10 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
 }
 
 // z = Q (x - o) for the 32 individuals of a tile (x: the lane's parity half)
-// followed by the objective.  4 KB unconditional MFMA steps per output tile.
+// followed by the objective.  One MFMA k-step per gene pair (2 p, 2 p + 1), p < Dh:
+// the steps past the dimension would add q * 0 products of the zero padding, which
+// leave the accumulator unchanged (it is never -0), so they are skipped (a uniform
+// branch, taken only in the last blocks).
 template <int KB, int FUNC>
 __device__ __forceinline__ float evaluate_tile(const float (&x)[4 * KB], const Geo& g, const Lds& L, int col, int h) {
     using S = Shape<KB>;
@@ -217,10 +220,12 @@ __device__ __forceinline__ float evaluate_tile(const float (&x)[4 * KB], const G
         const float4 o4 = lds4(orow + 4 * sb);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const float w = x[4 * sb + e] - f4(o4, e);
+            if (sb < KB - 1 || 4 * sb + e < g.Dh) {
+                const float w = x[4 * sb + e] - f4(o4, e);
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt)
-                acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), w, acc[jt], 0, 0, 0);
+                for (int jt = 0; jt < NT; ++jt)
+                    acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), w, acc[jt], 0, 0, 0);
+            }
         }
     }
     return objective<NT, FUNC>(acc, g.D, L.tab, h);
@@ -838,6 +843,7 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
     const int tid = threadIdx.x, lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
     Geo g{};
     g.D = td.dim;
+    g.Dh = (g.D + 1) >> 1;
     g.pop = td.pop;
     g.popP = (g.pop + 31) & ~31;
     Lds L{};
@@ -1141,8 +1147,9 @@ __global__ __launch_bounds__(2 * ObjShape<KB>::PB) void objective_kernel(ObjArgs
         const int d = 2 * s + h;
         x[s] = (d < D) ? xr[d] : 0.0f;
     }
-    Geo g;
+    Geo g{};
     g.D = D;
+    g.Dh = (D + 1) >> 1;
     Lds L;
     L.Q = Q;
     L.o = o;
