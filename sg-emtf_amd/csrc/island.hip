@@ -317,16 +317,19 @@ __device__ __forceinline__ DeMates de_mates(const EpochArgs& a, const Geo& g, co
 __device__ __forceinline__ void de_block(float (&t)[4], const float4& vb, const float4& v2, const float4& v3,
                                          const float4& vi, const U4& cb, int sb, int h, int jrand, int D, float F,
                                          float lo, float hi, uint32_t tcr) {
+    // gene d = 2 s + h: the tests d == jrand and d < D are made on 2 s against the
+    // lane's jrand - h and D - h, so no per-gene index is held in a register
+    const int jr = jrand - h, dl = D - h;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const int s = 4 * sb + e, d = 2 * s + h;
+        const int s2 = 2 * (4 * sb + e);
         const float diff = f4(v2, e) - f4(v3, e);
         const float sc = F * diff;
         float v = f4(vb, e) + sc;
         v = clamp_box(v, lo, hi);
         const uint32_t u16 = bits16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
-        const float tv = (u16 >= tcr && d != jrand) ? f4(vi, e) : v;
-        t[e] = (8 * sb + 8 <= D || d < D) ? tv : 0.0f;  // only a block reaching past D is masked
+        const float tv = (u16 >= tcr && s2 != jr) ? f4(vi, e) : v;
+        t[e] = (8 * sb + 8 <= D || s2 < dl) ? tv : 0.0f;  // only a block reaching past D is masked
     }
 }
 
@@ -367,9 +370,10 @@ __device__ __forceinline__ void ga_block(float (&t)[4], const float4& va, const 
     const U4 pm0 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb));
     const U4 pm1 = rng.block((uint32_t)gen, P_GA_PM, (uint32_t)ie, (uint32_t)h, (uint32_t)(2 * sb + 1));
     const float lo = c0.lo, hi = c0.hi, rng_ = c0.rng;
+    const int dlim = D - h;  // gene 2 s + h < D  <=>  2 s < D - h
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const int s = 4 * sb + e, d = 2 * s + h;
+        const int s2 = 2 * (4 * sb + e);
         const float p = uniform_open(pick(sbk, (uint32_t)e));
         const float tv = (p <= 0.5f) ? (2.0f * p) : (0.5f / (1.0f - p));
         float beta = pow_f(tv, c0.ec);
@@ -410,7 +414,7 @@ __device__ __forceinline__ void ga_block(float (&t)[4], const float4& va, const 
             c = c + step;
             c = clamp_box(c, lo, hi);
         }
-        t[e] = (8 * sb + 8 <= D || d < D) ? c : 0.0f;
+        t[e] = (8 * sb + 8 <= D || s2 < dlim) ? c : 0.0f;
     }
 }
 
