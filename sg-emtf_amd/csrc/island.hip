@@ -333,30 +333,6 @@ __device__ __forceinline__ void de_block(float (&t)[4], const float4& vb, const 
     }
 }
 
-template <int KB>
-__device__ __forceinline__ void trial_de(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
-                                         const Lds& L, const Rng& rng, int gen, int ie, int h) {
-    using S = Shape<KB>;
-    const int il = ie % g.m, base = ie - il;
-    const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
-    const DeMates mt = de_mates(a, g, rng, gen, ie);
-    const int off = h * S::HS;
-    const float* xb = L.X + (base + (rand1 ? mt.r1 : il)) * S::RS + off;
-    const float* x2 = L.X + (base + mt.r2) * S::RS + off;
-    const float* x3 = L.X + (base + mt.r3) * S::RS + off;
-    const float* xi = L.X + ie * S::RS + off;
-    U4 cb{0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int sb = 0; sb < KB; ++sb) {
-        if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
-        const float4 vb = lds4(xb + 4 * sb), v2 = lds4(x2 + 4 * sb), v3 = lds4(x3 + 4 * sb), vi = lds4(xi + 4 * sb);
-        float t[4];
-        de_block(t, vb, v2, v3, vi, cb, sb, h, mt.jrand, g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) tr[4 * sb + e] = t[e];
-    }
-}
-
 // Genes 2 s + h, s = 4 sb .. 4 sb + 3, of a GA child: SBX of parents va, vb
 // (spread sign by the child's parity in its pair) then polynomial mutation,
 // each clamped to the box, zero past the dimension.
@@ -430,27 +406,76 @@ __device__ __forceinline__ GaConst ga_const(const EpochArgs& a, const TaskDev& t
     return c;
 }
 
-template <int KB>
-__device__ __forceinline__ void trial_ga(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
-                                         const Lds& L, const Rng& rng, int gen, int ie, int h) {
+// The lane's trial half-row, block by block, each block's four genes fed straight into
+// the MFMA k-steps of z = Q (x - o): the next block's draws, loads and arithmetic are
+// independent of the matrix core's work and can issue while it runs.  The same genes,
+// k order and accumulators as generating the whole row first and then evaluating it.
+// The trial genes are left in tr (for the selection copy); returns f of the trial.
+template <int KB, int FUNC, int ALGO>
+__device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
+                                            const Lds& L, const Rng& rng, int gen, int ie, int col, int h) {
     using S = Shape<KB>;
-    const int m = g.m, il = ie % m, base = ie - il;
-    const U4 mb = rng.block((uint32_t)gen, P_GA_MATE, (uint32_t)(ie & ~1), 0u, 0u);
-    const int ra = (int)g.modM.mod(pick(mb, (uint32_t)(il & 1)));
-    const int rb = (int)g.modM.mod(pick(mb, (uint32_t)((il & 1) ^ 1)));
+    constexpr int NT = S::NT;
     const int off = h * S::HS;
-    const float* xa = L.X + (base + ra) * S::RS + off;
-    const float* xb = L.X + (base + rb) * S::RS + off;
-    const GaConst c0 = ga_const(a, td);
-    const bool odd = (il & 1) != 0;
+    const int il = ie % g.m, base = ie - il;
+    f16v acc[NT];
+#pragma unroll
+    for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+    const float* qrow = L.Q + col * S::RS + off;
+    const float* orow = L.o + off;
+    // GA: the pair's mates and constants; DE: the three mates and jrand
+    const float *p0, *p1, *p2, *p3;
+    int jrand = 0;
+    bool odd = false;
+    GaConst c0{};
+    if constexpr (ALGO == A_GA_SBX_PM) {
+        const U4 mb = rng.block((uint32_t)gen, P_GA_MATE, (uint32_t)(ie & ~1), 0u, 0u);
+        const int ra = (int)g.modM.mod(pick(mb, (uint32_t)(il & 1)));
+        const int rb = (int)g.modM.mod(pick(mb, (uint32_t)((il & 1) ^ 1)));
+        p0 = L.X + (base + ra) * S::RS + off;
+        p1 = L.X + (base + rb) * S::RS + off;
+        p2 = p3 = p0;
+        c0 = ga_const(a, td);
+        odd = (il & 1) != 0;
+    } else {
+        const bool rand1 = (a.p.algo == A_DE_RAND_1_BIN);
+        const DeMates mt = de_mates(a, g, rng, gen, ie);
+        p0 = L.X + (base + (rand1 ? mt.r1 : il)) * S::RS + off;
+        p1 = L.X + (base + mt.r2) * S::RS + off;
+        p2 = L.X + (base + mt.r3) * S::RS + off;
+        p3 = L.X + ie * S::RS + off;
+        jrand = mt.jrand;
+    }
+    U4 cb{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int sb = 0; sb < KB; ++sb) {
-        const float4 va = lds4(xa + 4 * sb), vb = lds4(xb + 4 * sb);
         float t[4];
-        ga_block(t, va, vb, rng, gen, ie, sb, h, odd, g.D, c0);
+        if constexpr (ALGO == A_GA_SBX_PM) {
+            ga_block(t, lds4(p0 + 4 * sb), lds4(p1 + 4 * sb), rng, gen, ie, sb, h, odd, g.D, c0);
+        } else {
+            if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
+            de_block(t, lds4(p0 + 4 * sb), lds4(p1 + 4 * sb), lds4(p2 + 4 * sb), lds4(p3 + 4 * sb), cb, sb, h, jrand,
+                     g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
+        }
+        float4 q[NT];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) tr[4 * sb + e] = t[e];
+        for (int jt = 0; jt < NT; ++jt) q[jt] = lds4(qrow + 32 * jt * S::RS + 4 * sb);
+        const float4 o4 = lds4(orow + 4 * sb);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            tr[4 * sb + e] = t[e];
+            // k-steps past the dimension add q * 0 of the zero padding: skipped
+            if (sb < KB - 1 || 4 * sb + e < g.Dh) {
+                const float w = t[e] - f4(o4, e);
+#pragma unroll
+                for (int jt = 0; jt < NT; ++jt)
+                    acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q[jt], e), w, acc[jt], 0, 0, 0);
+            }
+        }
     }
+    return objective<NT, FUNC>(acc, g.D, L.tab, h);
 }
 
 template <int KB>
@@ -503,9 +528,7 @@ __device__ __forceinline__ void generation_step(const EpochArgs& a, const TaskDe
         const int i = it * 32 + col;
         const bool valid = i < g.pop;
         const int ie = valid ? i : 0;
-        if constexpr (ALGO == A_GA_SBX_PM) trial_ga<KB>(tr, a, td, g, L, rng, gen, ie, h);
-        else trial_de<KB>(tr, a, td, g, L, rng, gen, ie, h);
-        const float f = evaluate_tile<KB, FUNC>(tr, g, L, col, h);
+        const float f = trial_eval<KB, FUNC, ALGO>(tr, a, td, g, L, rng, gen, ie, col, h);
         const bool accept = valid && (L.fit[i] >= f);
         if (multi) {
             if (accept) store_half_row<KB>(a.scratch + td.scr_off + (int64_t)i * S::RS + h * S::HS, tr);
