@@ -297,7 +297,32 @@ __device__ __forceinline__ DeMates de_mates(const EpochArgs& a, const Geo& g, co
     r.r3 = (int)g.modM.mod(b0.z);
     const bool ok = rand1 ? (m >= 4 && r.r1 != il && r.r2 != il && r.r2 != r.r1 && r.r3 != il && r.r3 != r.r1 && r.r3 != r.r2)
                           : (m >= 3 && r.r2 != r.r1 && r.r3 != r.r1 && r.r3 != r.r2);
-    if (!ok) {
+    bool replay = !ok;
+    if (!ok && m >= 4) {
+        // the rejection draws over the first seven words of the stream (block 0's x, y,
+        // z, then block 1) in straight-line code: each word is taken by the first mate
+        // still missing unless it collides with an excluded index; with m >= 4 every draw
+        // has an admissible value, so this is the sequential rejection draw whenever
+        // seven words suffice (a lane needing more replays it below)
+        const U4 b1 = rng.block((uint32_t)gen, P_DE_IDX, (uint32_t)ie, 0u, 1u);
+        const uint32_t w[7] = {b0.x, b0.y, b0.z, b1.x, b1.y, b1.z, b1.w};
+        int got = 0, q1 = 0, q2 = 0, q3 = 0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int v = (k == 0) ? r.r1 : ((k == 1) ? r.r2 : ((k == 2) ? r.r3 : (int)g.modM.mod(w[k])));
+            const bool bad = (rand1 && v == il) || (got >= 1 && v == q1) || (got >= 2 && v == q2);
+            const bool take = !bad && got < 3;
+            q1 = (take && got == 0) ? v : q1;
+            q2 = (take && got == 1) ? v : q2;
+            q3 = (take && got == 2) ? v : q3;
+            got += take ? 1 : 0;
+        }
+        r.r1 = q1;
+        r.r2 = q2;
+        r.r3 = q3;
+        replay = got < 3;
+    }
+    if (replay) {
         IdxStream st{b0.x, b0.y, b0.z, 0u, 0, 3, 0};
         int ex[3];
         int nex = 0;

@@ -207,3 +207,19 @@ def test_wide_q_in_lds(oracle_mod, algo):
     gpu.run(7)
     orc.run(7)
     _compare(gpu, orc, f"wide Q in LDS algo {algo}")
+
+
+@pytest.mark.parametrize("algo", [emtf.DE_RAND_1_BIN, emtf.DE_CURRENT_1_BIN])
+@pytest.mark.parametrize("m", [4, 5, 8])
+def test_mate_rejection_small_islands(oracle_mod, algo, m):
+    """Islands of m = 4, 5, 8 individuals: most mate draws collide, so the straight-line
+    seven-word rejection and its sequential replay (more than seven words) both run, in
+    the LDS and the wide kernel."""
+    tasks = tg.benchmark_tasks(3, 40, 8 * m, rotate=True, seed=20 + m) + \
+        tg.benchmark_tasks(2, 150, 8 * m, rotate=True, seed=30 + m)
+    params = dict(seed=m, algo=algo, n_islands=8, exchange_interval=3, transfer_interval=4, transfer_count=2)
+    gpu, orc = _pair(tasks, oracle_mod, **params)
+    for step in (5, 6):
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"m={m} algo={algo} gen {orc.generation}")
