@@ -35,7 +35,9 @@ def parse():
     ap.add_argument("--interval", type=int, default=20, help="generations per epoch = migration interval")
     ap.add_argument("--elites", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-gens", type=int, default=100, help="generations of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-gens", type=int, default=None,
+                    help="generations of the CPU-baseline sample (0 = skip; default: 200 for the plain workload, "
+                         "100 otherwise -- 10 to 40 s of CPU work on 16 threads)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
     ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "cvrp", "polish", "online"],
                     help="plain: the rotated continuous workload (default); uneven: 4096 tasks with "
@@ -267,6 +269,8 @@ def bench_online(a):
 
 def main():
     a = parse()
+    if a.cpu_gens is None:
+        a.cpu_gens = 200 if a.workload == "plain" else 100
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
         return

@@ -120,8 +120,9 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
     } else {
         // SKIP: groups of eight rows past the dimension are skipped outright (a
         // uniform branch) for the objectives with transcendental terms -- worth it
-        // where the row tiles are padded far past D (the wide kernel); rows j >= D
-        // are masked in every case
+        // where the row tiles are padded far past D (the wide kernel) or the terms are
+        // costly (Weierstrass, Schwefel in the LDS kernel); rows j >= D are masked in
+        // every case
         constexpr bool skip = SKIP && FUNC != F_SPHERE && FUNC != F_ELLIPTIC;
 #pragma unroll
         for (int jt = 0; jt < NT; ++jt)
@@ -500,7 +501,9 @@ __device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs
             }
         }
     }
-    return objective<NT, FUNC>(acc, g.D, L.tab, h);
+    // the padded row groups are skipped for the two costliest epilogues only (measured:
+    // the branch costs more than it saves for the others)
+    return objective<NT, FUNC, FUNC == F_WEIERSTRASS || FUNC == F_SCHWEFEL>(acc, g.D, L.tab, h);
 }
 
 template <int KB>
