@@ -269,6 +269,10 @@ def bench_online(a):
 
 def main():
     a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine shards tasks across "
+              f"GPUs only by design, see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
     if a.cpu_gens is None:
         a.cpu_gens = 200 if a.workload == "plain" else 100
     if a.calibrate_copy:
@@ -283,10 +287,6 @@ def main():
     if a.workload == "online":
         bench_online(a)
         return
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine shards tasks across "
-              f"GPUs only by design, see the sharding notes)", file=sys.stderr)
-        sys.exit(2)
     import emtf
     from emtf import hip
     from emtf import tasks as tg
