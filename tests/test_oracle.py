@@ -356,3 +356,61 @@ def test_ga_children_in_box(oracle_mod):
         lo, hi = tasks[t]["lo"], tasks[t]["hi"]
         assert np.all(x >= lo) and np.all(x <= hi)
         assert np.array_equal(oracle_mod.eval32(tasks[t], x).view(np.uint32), f.view(np.uint32))
+
+
+# --------------------------------------------- toy generation with draw tables
+def _toy():
+    d = json.load(open(os.path.join(GOLD, "toy_generation.json")))
+
+    def arr(v, shape=None):
+        a = np.array(v, np.uint32).view(np.float32)
+        return a.reshape(shape) if shape else a
+    task = dict(func=d["func"], dim=d["dim"], pop=d["pop"], xform=0, lo=d["lo"], hi=d["hi"], shift=arr(d["shift"]))
+    return d, task, arr(d["x"], (d["pop"], d["dim"])), arr(d["fitness"]), arr
+
+
+def _toy_params(d, mode):
+    base = dict(seed=d["seed"], n_islands=1)
+    if mode == "de":
+        return dict(base, algo=0, F=d["de"]["F"], CR=d["de"]["CR"])
+    return dict(base, algo=2, eta_c=d["ga"]["eta_c"], eta_m=d["ga"]["eta_m"], pm=d["ga"]["pm"])
+
+
+def test_toy_generation_draw_tables(oracle_mod):
+    """The fixture's draw tables are the oracle's Philox blocks at the documented addresses."""
+    d, task, X, fit, arr = _toy()
+    seed, gen = d["seed"], d["generation"]
+    for i, t in enumerate(d["de"]["draws"]):
+        b0 = oracle_mod.draw_block(seed, 0, gen, 2, i, 0, 0)
+        stream = b0[:3] + sum((oracle_mod.draw_block(seed, 0, gen, 2, i, 0, k) for k in (1, 2, 3)), [])
+        assert t["index_words"] == stream[:len(t["index_words"])] and t["jrand"] == b0[3] % d["dim"]
+    for i, t in enumerate(d["ga"]["draws"]):
+        mb = oracle_mod.draw_block(seed, 0, gen, 4, i & ~1, 0, 0)
+        assert t["mates"] == [mb[i & 1] % d["pop"], mb[(i & 1) ^ 1] % d["pop"]]
+
+
+@pytest.mark.parametrize("mode", ["de", "ga"])
+def test_toy_generation_oracle(oracle_mod, mode):
+    """One generation of the oracle from the toy population equals the numpy restatement
+    of the reference operators (tests/golden/make_toy_generation.py) bit for bit."""
+    d, task, X, fit, arr = _toy()
+    eng = oracle_mod.Engine([task], **_toy_params(d, mode))
+    eng.set_population(0, X, fit)
+    eng.run(1)
+    x1, f1 = eng.population(0)
+    assert np.array_equal(x1.view(np.uint32), np.array(d[mode]["x"], np.uint32).reshape(x1.shape))
+    assert np.array_equal(f1.view(np.uint32), np.array(d[mode]["fitness"], np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["de", "ga"])
+def test_toy_generation_gpu(mode):
+    """The same generation on the GPU equals the fixture."""
+    import emtf
+    d, task, X, fit, arr = _toy()
+    eng = emtf.Engine([task], **_toy_params(d, mode))
+    eng.set_population(0, X, fit)
+    eng.run(1)
+    x1, f1 = eng.population(0)
+    assert np.array_equal(x1.view(np.uint32), np.array(d[mode]["x"], np.uint32).reshape(x1.shape))
+    assert np.array_equal(f1.view(np.uint32), np.array(d[mode]["fitness"], np.uint32))
