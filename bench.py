@@ -45,6 +45,9 @@ def parse():
                          "polish: time to mean error 1e-6, adjoint polish vs plain DE at D=100; "
                          "online: 10000 plain tasks admitted in 100 batches at U[0,100) ms gaps")
     ap.add_argument("--batches", type=int, default=100, help="online workload: arrival batches")
+    ap.add_argument("--algo", default="de", choices=["de", "ga"],
+                    help="plain / uneven workloads: DE/rand/1/bin (default) or the reference's GA operators "
+                         "(SBX eta_c 2 + polynomial mutation eta_m 5, pm 1)")
     ap.add_argument("--seeds", type=int, default=10, help="polish workload: seeds 0..N-1")
     ap.add_argument("--no-tune", action="store_true", help="switch the kernel-variant / stream autotuner off")
     ap.add_argument("--gens", type=int, default=3000, help="polish workload: generations per run")
@@ -298,8 +301,13 @@ def main():
     else:
         funcs = [emtf.FUNC_IDS[a.only_func]] * a.tasks if a.only_func else None
         tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed, funcs=funcs)
-    params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval,
-                  transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0)
+    if a.algo == "ga":
+        params = dict(algo=emtf.GA_SBX_PM, eta_c=2.0, eta_m=5.0, pm=1.0, transfer_interval=a.interval,
+                      transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0)
+    else:
+        params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval,
+                      transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0)
+    algo_name = "GA SBX(eta_c 2)+PM(eta_m 5, pm 1)" if a.algo == "ga" else "DE/rand/1/bin F=0.5 CR=0.9"
     eng = emtf.Engine(tasks, **params)
     for _ in range(a.warmup):
         eng.run(a.interval)
@@ -327,7 +335,8 @@ def main():
 
     traffic = None
     tp = os.path.join(ROOT, "profiles", "island_epoch_traffic.json")
-    if not uneven and not a.only_func and (a.tasks, a.dim, a.pop, a.interval) == (1024, 50, 128, 20) and os.path.exists(tp):
+    if (not uneven and not a.only_func and a.algo == "de" and (a.tasks, a.dim, a.pop, a.interval) == (1024, 50, 128, 20)
+            and os.path.exists(tp)):
         traffic = json.load(open(tp))["hbm_bytes_per_launch"]
 
     cpu = None
@@ -341,7 +350,7 @@ def main():
         "config": {"workload": (f"batched uneven: {a.tasks} rotated tasks, D~U{{10..200}}, pop~U{{32..256}} rounded to 32"
                                 if uneven else
                                 f"plain GPU: {a.tasks} rotated tasks{' (all ' + a.only_func + ')' if a.only_func else ''}, D={a.dim}, pop={a.pop}")
-                               + f", DE/rand/1/bin F=0.5 CR=0.9, {a.elites} elites every {a.interval} gens, "
+                               + f", {algo_name}, {a.elites} elites every {a.interval} gens, "
                                f"{a.interval * a.steps} timed generations",
                    "tasks": a.tasks, "dim": "U{10..200}" if uneven else a.dim, "pop": "U{32..256}" if uneven else a.pop,
                    "generations": a.interval * a.steps,
