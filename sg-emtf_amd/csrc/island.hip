@@ -439,8 +439,13 @@ __device__ __forceinline__ GaConst ga_const(const EpochArgs& a, const TaskDev& t
 // The trial genes are left in tr (for the selection copy); returns f of the trial.
 template <int KB, int FUNC, int ALGO>
 __device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
-                                            const Lds& L, const Rng& rng, int gen, int ie, int col, int h) {
+                                            const Lds& L, const Rng& rng, int gen, int ie, int col, int h_in) {
     using S = Shape<KB>;
+    // the lane's half index is made opaque here: otherwise the compiler hoists every
+    // mask and address derived from it out of the generation loop, where the lane masks
+    // outgrow the scalar registers and come back from spill slots each generation
+    int h = h_in;
+    asm volatile("" : "+v"(h));
     constexpr int NT = S::NT;
     const int off = h * S::HS;
     const int il = ie % g.m, base = ie - il;
@@ -1340,10 +1345,14 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
                                                 float* T, const Rng& rng, int gen, int tid, int nsb) {
     using S = WShape<NT>;
     constexpr int RS = S::RS, HS = S::HS;
-    const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
-    const float* qrow = L.Q + col * RS + h * HS;
-    const float* orow = L.o + h * HS;
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 31;
     for (int it = wave; it < g.ntiles; it += WT / 64) {
+        // the lane's half index is made opaque per tile (masks derived from it are not
+        // hoisted out of the generation loop into spilled scalar registers)
+        int h = lane >> 5;
+        asm volatile("" : "+v"(h));
+        const float* qrow = L.Q + col * RS + h * HS;
+        const float* orow = L.o + h * HS;
         const int i = it * 32 + col;
         const bool valid = i < g.pop;
         const int ie = valid ? i : 0;
