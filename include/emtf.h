@@ -88,7 +88,8 @@ typedef struct emtf_handle emtf_handle;
 int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params* params, emtf_handle** out);
 
 /* Advance every task by `generations` (asynchronous on `stream`, a hipStream_t;
- * NULL = default stream).  Replaces iter()'s loop of GeneticOpera launches and
+ * NULL = default stream; a run on another stream than the previous call's is ordered
+ * after that call's work).  Replaces iter()'s loop of GeneticOpera launches and
  * the knowledgeTransfer handshake (SG-EMTF.cu:1646-1745, kernel :1289). */
 int emtf_run(emtf_handle* h, int32_t generations, void* stream);
 

@@ -37,6 +37,7 @@ struct emtf_cvrp_handle {
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev_join;
     hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_order = nullptr;  // orders a run on a new stream after the previous one
     emtf::CvrpArgs a{};
     std::vector<emtf::CvrpDev> inst;
     emtf::CvrpDev* d_inst = nullptr;
@@ -59,6 +60,7 @@ static void cv_free(emtf_cvrp_handle* h) {
     for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
     for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_order) (void)hipEventDestroy(h->ev_order);
     delete h;
 }
 
@@ -237,7 +239,14 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
 int emtf_cvrp_run(emtf_cvrp_handle* h, int32_t generations, void* stream) {
     if (!h) return cfail(EMTF_E_INVALID, "handle is NULL");
     if (generations < 0) return cfail(EMTF_E_INVALID, "generations < 0");
-    h->stream = (hipStream_t)stream;
+    // a run on another stream than the previous call's waits for that call's work
+    if ((hipStream_t)stream != h->stream) {
+        if ((!h->ev_order && hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming) != hipSuccess) ||
+            hipEventRecord(h->ev_order, h->stream) != hipSuccess ||
+            hipStreamWaitEvent((hipStream_t)stream, h->ev_order, 0) != hipSuccess)
+            return cfail(EMTF_E_DEVICE, "stream ordering");
+        h->stream = (hipStream_t)stream;
+    }
     // epochs of at most 100 generations keep each launch short
     while (generations > 0) {
         const int g = std::min(generations, 100);

@@ -217,6 +217,7 @@ struct emtf_handle {
     std::vector<hipStream_t> side;  // extra streams for buckets 1.. (non-blocking, default priority)
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join;
+    hipEvent_t ev_order = nullptr;  // orders a run on a new stream after the previous one
     // convergence trace: best of every task at trace_g0 + k trace_interval, k < trace_max
     int trace_interval = 0, trace_max = 0, trace_points = 0, trace_g0 = 0;
     float* d_trace = nullptr;        // [trace_max][T]
@@ -241,7 +242,20 @@ static void free_handle(emtf_handle* h) {
 
     for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_order) (void)hipEventDestroy(h->ev_order);
     delete h;
+}
+
+// Work queued on a stream other than the previous call's waits for that call's work: a
+// handle's launches stay in call order whichever streams the caller passes.
+static int use_stream(emtf_handle* h, hipStream_t s) {
+    if (s != h->stream) {
+        if (!h->ev_order) HIP_OR_FAIL(hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming));
+        HIP_OR_FAIL(hipEventRecord(h->ev_order, h->stream));
+        HIP_OR_FAIL(hipStreamWaitEvent(s, h->ev_order, 0));
+        h->stream = s;
+    }
+    return EMTF_OK;
 }
 
 // Per-dimension table: Elliptic weights 10^(6 j/(D-1)), Griewank 1/(2 pi sqrt(j+1)).
@@ -795,7 +809,7 @@ int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
     if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
     if (generations < 0) return fail(EMTF_E_INVALID, "generations < 0");
     hipStream_t s = (hipStream_t)stream;
-    h->stream = s;
+    if (const int rc = use_stream(h, s); rc != EMTF_OK) return rc;
     const int TI = h->p.transfer_interval;
     const bool xfer = TI > 0 && h->T > 1 && h->p.transfer_count > 0;
     const int g_end = h->gen + generations;
@@ -837,8 +851,9 @@ int emtf_trace_start(emtf_handle* h, int32_t interval, int32_t max_points, void*
         return EMTF_OK;
     }
     if (max_points < 1) return fail(EMTF_E_INVALID, "max_points < 1");
-    h->stream = (hipStream_t)stream;
+    // the previous run's stream first: its trace kernels may still write the old buffer
     HIP_OR_FAIL(hipStreamSynchronize(h->stream));
+    if (const int rc = use_stream(h, (hipStream_t)stream); rc != EMTF_OK) return rc;
     (void)hipFree(h->d_trace);
     (void)hipFree(h->d_trace_worst);
     h->d_trace = h->d_trace_worst = nullptr;

@@ -223,3 +223,18 @@ def test_mate_rejection_small_islands(oracle_mod, algo, m):
         gpu.run(step)
         orc.run(step)
         _compare(gpu, orc, f"m={m} algo={algo} gen {orc.generation}")
+
+
+def test_runs_on_changing_streams_stay_ordered(oracle_mod):
+    """Consecutive runs queued on different streams without host synchronisation give the
+    same populations as one stream (each run waits for the previous call's work)."""
+    from emtf import hip
+    tasks = tg.benchmark_tasks(12, 50, 128, rotate=True, seed=21)
+    params = dict(seed=4, transfer_interval=5, transfer_count=2)
+    gpu, orc = _pair(tasks, oracle_mod, **params)
+    streams = [hip.Stream() for _ in range(3)]
+    for k in range(9):
+        gpu.run(5, stream=streams[k % 3].handle)
+    orc.run(45)
+    _compare(gpu, orc, "45 generations over three streams")
+    gpu.close()
