@@ -915,28 +915,9 @@ int emtf_set_population(emtf_handle* h, int32_t task, const float* x, const floa
 int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_stride) {
     if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
     if (x_best && x_stride < h->max_dim) return fail(EMTF_E_INVALID, "x_stride < max dim");
-    HIP_OR_FAIL(hipStreamSynchronize(h->stream));
-    const emtf::TaskDev& last = h->tasks.back();
-    std::vector<float> f((size_t)(last.fit_off + last.pop));
-    HIP_OR_FAIL(hipMemcpy(f.data(), h->d_fit, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
-    std::vector<float> x;
-    if (x_best) {
-        x.resize((size_t)(last.x_off + (int64_t)last.pop * last.dim));
-        HIP_OR_FAIL(hipMemcpy(x.data(), h->d_X, sizeof(float) * x.size(), hipMemcpyDeviceToHost));
-    }
-    for (int t = 0; t < h->T; ++t) {
-        const emtf::TaskDev& td = h->tasks[t];
-        const float* ft = f.data() + td.fit_off;
-        int best = 0;
-        for (int i = 1; i < td.pop; ++i) {
-            const float fb = std::isnan(ft[best]) ? INFINITY : ft[best];
-            const float fi = std::isnan(ft[i]) ? INFINITY : ft[i];
-            if (fi < fb) best = i;
-        }
-        if (fitness) fitness[t] = ft[best];
-        if (x_best) std::memcpy(x_best + (size_t)t * x_stride, x.data() + td.x_off + (int64_t)best * td.dim, sizeof(float) * td.dim);
-    }
-    return EMTF_OK;
+    // the first of the device ranking by (fitness, index) with NaN last: the lowest index
+    // among the smallest fitness values
+    return emtf_get_elites(h, 1, fitness, x_best, x_stride);
 }
 
 static int objective_call(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
