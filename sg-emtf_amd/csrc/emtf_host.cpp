@@ -222,6 +222,9 @@ struct emtf_handle {
     int trace_interval = 0, trace_max = 0, trace_points = 0, trace_g0 = 0;
     float* d_trace = nullptr;        // [trace_max][T]
     float* d_trace_worst = nullptr;  // [T], the worst fitness at trace_g0
+    // elite export staging, kept between calls (grown on demand)
+    float* d_elite = nullptr;
+    size_t elite_cap = 0;        // floats
 };
 
 static void free_handle(emtf_handle* h) {
@@ -238,6 +241,7 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_order);
     (void)hipFree(h->d_trace);
     (void)hipFree(h->d_trace_worst);
+    (void)hipFree(h->d_elite);
     for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
 
     for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
@@ -982,18 +986,24 @@ int emtf_get_elites(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t
     if (x && x_stride < h->max_dim) return fail(EMTF_E_INVALID, "x_stride < max dim");
     for (const emtf::TaskDev& td : h->tasks)
         if (k > td.pop) return fail(EMTF_E_INVALID, "k exceeds a task's population");
-    float *dx = nullptr, *df = nullptr;
     const size_t nx = x ? (size_t)h->T * k * x_stride : 0, nf = (size_t)h->T * k;
-    hipError_t e = hipMalloc((void**)&df, nf * sizeof(float));
-    if (e == hipSuccess && x) e = hipMalloc((void**)&dx, nx * sizeof(float));
+    hipError_t e = hipSuccess;
+    if (nf + nx > h->elite_cap) {
+        e = hipStreamSynchronize(h->stream);  // a previous export may still read the old buffer
+        (void)hipFree(h->d_elite);
+        h->d_elite = nullptr;
+        h->elite_cap = 0;
+        if (e == hipSuccess) e = hipMalloc((void**)&h->d_elite, (nf + nx) * sizeof(float));
+        if (e == hipSuccess) h->elite_cap = nf + nx;
+    }
+    float* df = h->d_elite;
+    float* dx = x ? h->d_elite + nf : nullptr;
     if (e == hipSuccess && x) e = hipMemsetAsync(dx, 0, nx * sizeof(float), h->stream);
     if (e == hipSuccess)
         e = (hipError_t)emtf_launch_elites(h->d_tasks, h->T, h->d_X, h->d_fit, k, x_stride, dx, df, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && fitness) e = hipMemcpy(fitness, df, nf * sizeof(float), hipMemcpyDeviceToHost);
     if (e == hipSuccess && x) e = hipMemcpy(x, dx, nx * sizeof(float), hipMemcpyDeviceToHost);
-    (void)hipFree(dx);
-    (void)hipFree(df);
     if (e != hipSuccess) return fail(EMTF_E_DEVICE, std::string("elite export: ") + hipGetErrorString(e));
     return EMTF_OK;
 }
