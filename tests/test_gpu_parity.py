@@ -238,3 +238,17 @@ def test_runs_on_changing_streams_stay_ordered(oracle_mod):
     orc.run(45)
     _compare(gpu, orc, "45 generations over three streams")
     gpu.close()
+
+
+@pytest.mark.parametrize("algo", [emtf.DE_RAND_1_BIN, emtf.DE_CURRENT_1_BIN, emtf.GA_SBX_PM])
+def test_smallest_shapes(oracle_mod, algo):
+    """D = 1, 2, 3 with populations of 2 to 8 (islands too small for three distinct mates
+    take the deterministic fallback of the rejection draw), every objective, transfers."""
+    tasks = (tg.benchmark_tasks(8, 1, 2, rotate=False, seed=31) + tg.benchmark_tasks(8, 2, 4, rotate=True, seed=32)
+             + tg.benchmark_tasks(8, 3, 8, rotate=True, seed=33))
+    params = dict(seed=7, algo=algo, transfer_interval=3, transfer_count=1)
+    gpu, orc = _pair(tasks, oracle_mod, **params)
+    for step in (1, 4, 6):
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"small shapes algo {algo} gen {orc.generation}")
