@@ -252,3 +252,27 @@ def test_smallest_shapes(oracle_mod, algo):
         gpu.run(step)
         orc.run(step)
         _compare(gpu, orc, f"small shapes algo {algo} gen {orc.generation}")
+
+
+def test_cpu_plain_configuration_full_run(oracle_mod):
+    """The CPU-plain configuration in full: 8 unrotated tasks (the eight objectives), D = 50,
+    pop 64, DE/rand/1/bin F = 0.5 CR = 0.9, 1000 generations, 2 elites from a random other
+    task every 10 generations; bit-exact at generations 10, 500 and 1000."""
+    tasks = tg.benchmark_tasks(8, 50, 64, rotate=False, seed=0)
+    params = dict(seed=0, algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=10, transfer_count=2)
+    gpu, orc = _pair(tasks, oracle_mod, **params)
+    for step in (10, 490, 500):
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"CPU-plain configuration gen {orc.generation}")
+
+
+def test_plain_gpu_configuration_full_run_subset(oracle_mod):
+    """64 tasks of the plain GPU configuration (rotated, D = 50, pop 128, 4 elites every 20
+    generations) over all 2000 generations, bit-exact at the end."""
+    tasks = tg.benchmark_tasks(64, 50, 128, rotate=True, seed=0)
+    params = dict(seed=0, algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=20, transfer_count=4)
+    gpu, orc = _pair(tasks, oracle_mod, **params)
+    gpu.run(2000)
+    orc.run(2000, threads=8)
+    _compare(gpu, orc, "plain GPU configuration, 64 tasks, gen 2000")
