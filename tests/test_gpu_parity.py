@@ -276,3 +276,48 @@ def test_plain_gpu_configuration_full_run_subset(oracle_mod):
     gpu.run(2000)
     orc.run(2000, threads=8)
     _compare(gpu, orc, "plain GPU configuration, 64 tasks, gen 2000")
+
+
+def _fitness_matches_fresh_evaluation(gpu, tasks, ids):
+    """Stored fitness of every individual equals a fresh device evaluation of its row."""
+    import torch
+    from emtf.objective import forward_raw
+    for t in ids:
+        task = tasks[t]
+        x, f = gpu.population(t)
+        rot = torch.from_numpy(np.ascontiguousarray(task["rot"], np.float32)).cuda() if "rot" in task else None
+        shift = torch.from_numpy(np.ascontiguousarray(task["shift"], np.float32)).cuda()
+        fresh = forward_raw(task["func"], torch.from_numpy(x).cuda(), rot, shift).cpu().numpy()
+        assert np.array_equal(fresh.view(np.uint32), f.view(np.uint32)), f"task {t}: stored fitness differs"
+
+
+def test_full_size_plain_properties():
+    """The plain GPU configuration at its real size (1024 tasks, 2000 generations): every
+    task's best never increases (device trace every 100 generations) and every stored
+    fitness equals a fresh evaluation of its row."""
+    tasks = tg.benchmark_tasks(1024, 50, 128, rotate=True, seed=0)
+    gpu = emtf.Engine(tasks, seed=0, algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=20, transfer_count=4)
+    gpu.trace_start(100, 21)
+    gpu.run(2000)
+    rows = gpu.trace()
+    assert rows.shape == (1024, 22)
+    assert np.all(np.diff(rows[:, :-1], axis=1) <= 0)
+    assert np.all(rows[:, -1] >= rows[:, 0])
+    _fitness_matches_fresh_evaluation(gpu, tasks, range(0, 1024, 7))
+    gpu.close()
+
+
+def test_full_size_uneven_properties():
+    """The batched uneven configuration at its real size (4096 tasks, D ~ U{10..200}, pop ~
+    U{32..256}), 200 generations: best never increases, stored fitness equals a fresh
+    evaluation (sampled tasks with D <= 128, the objective kernel's range)."""
+    tasks = tg.uneven_tasks(4096, seed=0)
+    gpu = emtf.Engine(tasks, seed=0, algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=20, transfer_count=4)
+    gpu.trace_start(20, 11)
+    gpu.run(200)
+    rows = gpu.trace()
+    assert np.all(np.diff(rows[:, :-1], axis=1) <= 0)
+    ids = [t for t in range(0, 4096, 31) if tasks[t]["dim"] <= 128]
+    assert len(ids) > 50
+    _fitness_matches_fresh_evaluation(gpu, tasks, ids)
+    gpu.close()
