@@ -152,3 +152,28 @@ def test_gpu_cvrp_bit_exact_32bit_labels(oracle_mod):
         to, co = orc.population(i)
         assert np.array_equal(cg, co), i
         assert np.array_equal(tg, to), i
+
+
+@pytest.mark.gpu
+def test_gpu_cvrp_full_size_properties(oracle_mod):
+    """The CVRP benchmark batch at its real size (512 instances, n ~ U{50..200}, pop 128,
+    4 islands): over 500 generations each instance's best cost never increases, every tour
+    stays a permutation of the customers, and every stored cost equals the oracle's optimal
+    split of the stored tour (sampled instances)."""
+    from emtf.cvrp import CvrpEngine
+    inst = random_instances(512, seed=0)
+    gpu = CvrpEngine(inst, p_swap=0.3, p_2opt=0.3, n_islands=4, migrate_interval=25, migrate_count=2, seed=0)
+    prev = gpu.best()[0]
+    for _ in range(5):
+        gpu.run(100)
+        cur = gpu.best()[0]
+        assert np.all(cur <= prev)
+        prev = cur
+    for i in range(0, 512, 37):
+        ins = inst[i]
+        dist = oracle_mod.cvrp_dist_matrix(ins["xy"])
+        dem = ins["demand"].astype(np.uint8)
+        tours, costs = gpu.population(i)
+        for k in range(0, ins["pop"], 9):
+            assert sorted(tours[k].tolist()) == list(range(1, ins["n"] + 1))
+            assert costs[k] == oracle_mod.cvrp_split(tours[k], dist, dem, ins["capacity"]), (i, k)
