@@ -1,8 +1,10 @@
 /* abi_example.c -- a plain C99 caller of the engine's C ABI (include/emtf.h), the way a
  * reference-side host program would drive it: 16 shifted Sphere / Rastrigin tasks,
- * D = 20, pop 64, DE/rand/1/bin with elite transfer, 200 generations.  Prints one line
- * "ok <best fitness of task 0> <best fitness of task 1>" or the library's error message.
- * Built and run by tests/test_abi.py. */
+ * D = 20, pop 64, DE/rand/1/bin with elite transfer, 200 generations, with the best of
+ * every task recorded on the device every 50 generations.  Prints one line
+ * "ok <best of task 0> <best of task 1> <trace points> <task 0 at the last point>
+ * <second-best of task 0>" or the library's error message.  Built and run by
+ * tests/test_abi.py. */
 #include <stdio.h>
 #include <stdlib.h>
 #include "emtf.h"
@@ -38,13 +40,23 @@ int main(void) {
     }
     float best[T];
     static float xb[T][D];
-    if (emtf_run(h, 200, NULL) != EMTF_OK || emtf_sync(h) != EMTF_OK ||
-        emtf_get_best(h, best, &xb[0][0], D) != EMTF_OK) {
+    static float trace[5][T], worst[T];
+    static float ef[T][2], ex[T][2][D];
+    int32_t points = 0;
+    if (emtf_trace_start(h, 50, 5, NULL) != EMTF_OK || emtf_run(h, 200, NULL) != EMTF_OK ||
+        emtf_sync(h) != EMTF_OK || emtf_get_best(h, best, &xb[0][0], D) != EMTF_OK ||
+        emtf_trace(h, &points, &trace[0][0], worst) != EMTF_OK ||
+        emtf_get_elites(h, 2, &ef[0][0], &ex[0][0][0], D) != EMTF_OK) {
         printf("error %s\n", emtf_last_error());
         emtf_destroy(h);
         return 1;
     }
-    printf("ok %.9g %.9g\n", best[0], best[1]);
+    if (points != 5 || trace[4][0] != best[0] || ef[0][0] != best[0]) {
+        printf("error inconsistent exports\n");
+        emtf_destroy(h);
+        return 1;
+    }
+    printf("ok %.9g %.9g %d %.9g %.9g\n", best[0], best[1], (int)points, trace[4][0], ef[0][1]);
     emtf_destroy(h);
     return 0;
 }

@@ -141,6 +141,9 @@ def test_c_example_runs(tmp_path, oracle_mod):
     best = [np.float32(np.min(f)) for f in fo]
     # "%.9g" round-trips binary32
     assert [np.float32(v) for v in out[1:3]] == best
+    assert int(out[3]) == 5 and np.float32(out[4]) == best[0]
+    second = np.sort(fo[0])[1]  # the elite export's rank 1 of task 0
+    assert np.float32(out[5]) == second
 
 
 def test_trace_argument_checks():
