@@ -40,7 +40,7 @@ enum { EMTF_DE_RAND_1_BIN = 0, EMTF_DE_CURRENT_1_BIN = 1, EMTF_GA_SBX_PM = 2 };
  * Range (SG-EMTF.cu:92-124, rangeInit :197-214, para_init :181-195). */
 typedef struct emtf_task_desc {
     int32_t func;        /* EMTF_SPHERE .. EMTF_ELLIPTIC */
-    int32_t dim;         /* 1 .. 128 */
+    int32_t dim;         /* 1 .. 256 */
     int32_t pop;         /* individuals, a multiple of n_islands */
     int32_t xform;       /* EMTF_SHIFT / EMTF_ROTATE / EMTF_DIAG */
     float lo, hi;        /* search box, every dimension */
@@ -139,12 +139,12 @@ int emtf_destroy(emtf_handle* h);
 int emtf_eval_draws(uint64_t seed, int32_t n, const uint32_t* addrs, uint32_t* out, void* stream);
 
 /* ---- Objective and adjoint (device pointers, asynchronous on `stream`) -------
- * f[i] = f(Q (x_i - o)) for n points x[n][dim], bit for bit the engine's evaluator
- * (rot NULL: identity; shift NULL: zero).  Replaces the reference's evaluation()
+ * f[i] = f(Q (x_i - o)) for n points x[n][dim], dim 1 .. 256, bit for bit the engine's
+ * evaluator (rot NULL: identity; shift NULL: zero).  Replaces the reference's evaluation()
  * (proposed model/SG-EMTF_DE.cu:834-1069) as a reusable operator. */
 int emtf_objective_fwd(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot, const float* shift,
                        float* f, void* stream);
-/* gx[i] = gf[i] * Q^T (df/dz)(Q (x_i - o)): the hand-written adjoint in binary32.
+/* gx[i] = gf[i] * Q^T (df/dz)(Q (x_i - o)): the hand-written adjoint in binary32, dim 1 .. 128.
  * No counterpart in the reference (it has no gradients); used by the gradient polish
  * and by the torch.autograd.Function wrapper. */
 int emtf_objective_adjoint(int32_t func, int32_t dim, int32_t n, const float* x, const float* rot,

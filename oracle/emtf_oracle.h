@@ -116,6 +116,12 @@ int32_t or_generation(const or_engine* e);
 void or_get_population(const or_engine* e, int32_t task, float* x, float* fit);
 void or_set_population(or_engine* e, int32_t task, const float* x, const float* fit);
 void or_get_best(const or_engine* e, float* fit, float* x_best, int32_t x_stride);
+/* Single phases on the current state (tests pin them one at a time): the cross-island
+ * exchange of generation gen, the island migration, and one inter-task transfer event
+ * (every task's elites snapshotted, then every task receives its partner's). */
+void or_phase_exchange(or_engine* e, int32_t task, int32_t gen);
+void or_phase_migrate(or_engine* e, int32_t task);
+void or_phase_transfer(or_engine* e, int32_t gen);
 
 /* ---- CVRP (exact integers) -------------------------------------------------- */
 typedef struct {
@@ -140,6 +146,8 @@ void or_cvrp_child(const uint8_t* p1, const uint8_t* p2, int n, int a, int b, in
 or_cvrp* or_cvrp_create(const or_cvrp_instance* inst, int32_t n_inst, const or_cvrp_params* p);
 void or_cvrp_run(or_cvrp* e, int32_t gens, int32_t n_threads);
 void or_cvrp_get(const or_cvrp* e, int32_t inst, uint8_t* tours, int32_t* cost);
+void or_cvrp_set(or_cvrp* e, int32_t inst, const uint8_t* tours, const int32_t* cost);
+void or_cvrp_phase_migrate(or_cvrp* e, int32_t inst);
 void or_cvrp_destroy(or_cvrp* e);
 
 #ifdef __cplusplus

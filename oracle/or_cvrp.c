@@ -29,6 +29,13 @@
 
 #define CV_RING 64
 
+/* Planted mistake for the mutation tests (see or_engine.c): OR_MUTANT 7 sends rank j to
+ * destination rank m-1-j.  The oracle is built with OR_MUTANT undefined (0). */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+#define MUT_CV_DST(m, k, j) (OR_MUTANT == 7 ? (m) - 1 - (j) : (m) - (k) + (j))
+
 typedef struct {
     int n, cap, pop;
     uint8_t* dist;     /* (n+1) x (n+1) */
@@ -120,6 +127,25 @@ static void cv_rank(const int32_t* f, int n, int* order) {
     }
 }
 
+/* Island migration on a snapshot of the costs: rank j of island q replaces rank m-k+j of
+ * island q+1 (mod n) iff strictly cheaper, as the continuous engine's island_migration. */
+static void cv_migrate(or_cvrp* e, int ti) {
+    cv_state* s = &e->s[ti];
+    const int n = s->n, P = s->pop, nis = e->p.n_islands, m = P / nis, k = e->p.migrate_count;
+    for (int q = 0; q < nis; ++q) cv_rank(s->cost + q * m, m, s->order + q * m);
+    memcpy(s->ccost, s->cost, sizeof(int32_t) * (size_t)P);
+    for (int q = 0; q < nis; ++q) {
+        const int pd = (q + 1) % nis;
+        for (int j = 0; j < k; ++j) {
+            const int src = q * m + s->order[q * m + j], dst = pd * m + s->order[pd * m + MUT_CV_DST(m, k, j)];
+            if (s->ccost[src] < s->ccost[dst]) {
+                s->cost[dst] = s->ccost[src];
+                memcpy(s->P + (size_t)dst * n, s->P + (size_t)src * n, (size_t)n);
+            }
+        }
+    }
+}
+
 static void cv_generation(or_cvrp* e, int ti, int gen) {
     cv_state* s = &e->s[ti];
     const int n = s->n, P = s->pop, m = P / e->p.n_islands;
@@ -146,21 +172,9 @@ static void cv_generation(or_cvrp* e, int ti, int gen) {
             memcpy(s->P + (size_t)i * n, s->C + (size_t)i * n, (size_t)n);
         }
     }
-    const int nis = e->p.n_islands, k = e->p.migrate_count;
-    if (nis > 1 && e->p.migrate_interval > 0 && k > 0 && gen % e->p.migrate_interval == 0) {
-        for (int q = 0; q < nis; ++q) cv_rank(s->cost + q * m, m, s->order + q * m);
-        memcpy(s->ccost, s->cost, sizeof(int32_t) * (size_t)P);
-        for (int q = 0; q < nis; ++q) {
-            const int pd = (q + 1) % nis;
-            for (int j = 0; j < k; ++j) {
-                const int src = q * m + s->order[q * m + j], dst = pd * m + s->order[pd * m + m - k + j];
-                if (s->ccost[src] < s->ccost[dst]) {
-                    s->cost[dst] = s->ccost[src];
-                    memcpy(s->P + (size_t)dst * n, s->P + (size_t)src * n, (size_t)n);
-                }
-            }
-        }
-    }
+    if (e->p.n_islands > 1 && e->p.migrate_interval > 0 && e->p.migrate_count > 0 &&
+        gen % e->p.migrate_interval == 0)
+        cv_migrate(e, ti);
 }
 
 or_cvrp* or_cvrp_create(const or_cvrp_instance* inst, int32_t n_inst, const or_cvrp_params* p) {
@@ -227,6 +241,16 @@ void or_cvrp_get(const or_cvrp* e, int32_t inst, uint8_t* tours, int32_t* cost) 
     const cv_state* s = &e->s[inst];
     if (tours) memcpy(tours, s->P, (size_t)s->pop * s->n);
     if (cost) memcpy(cost, s->cost, sizeof(int32_t) * (size_t)s->pop);
+}
+
+void or_cvrp_phase_migrate(or_cvrp* e, int32_t inst) {
+    if (e->p.n_islands > 1 && e->p.migrate_count > 0) cv_migrate(e, inst);
+}
+
+void or_cvrp_set(or_cvrp* e, int32_t inst, const uint8_t* tours, const int32_t* cost) {
+    cv_state* s = &e->s[inst];
+    if (tours) memcpy(s->P, tours, (size_t)s->pop * s->n);
+    if (cost) memcpy(s->cost, cost, sizeof(int32_t) * (size_t)s->pop);
 }
 
 void or_cvrp_destroy(or_cvrp* e) {

@@ -35,6 +35,21 @@
 #include <omp.h>
 #endif
 
+/* Planted mistakes for the mutation tests (tests/test_oracle_pins.py builds this file with
+ * -DOR_MUTANT=k into a temporary directory and checks that the property tests reject the
+ * result).  The oracle itself is built with OR_MUTANT undefined, i.e. 0: every macro below
+ * is then the plain expression on its right. */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+#define MUT_XCH_INDEX(q) (OR_MUTANT == 1 ? (uint32_t)(q) : 0u)               /* offset per (island, slot) */
+#define MUT_MIG_DST(m, k, j) (OR_MUTANT == 2 ? (m) - 1 - (j) : (m) - (k) + (j)) /* destination rank */
+#define MUT_MIG_BETTER(a, b) (OR_MUTANT == 3 ? (a) <= (b) : (a) < (b))          /* strictly better */
+#define MUT_MIG_SRC(m, j) (OR_MUTANT == 4 ? (m) - 1 - (j) : (j))               /* source rank */
+#define MUT_XFER_RD(rd, rs) (OR_MUTANT == 5 ? (rs) : (rd))                    /* box map ranges swapped */
+#define MUT_XFER_RS(rd, rs) (OR_MUTANT == 5 ? (rd) : (rs))
+#define MUT_XFER_TGT(P, k, j) (OR_MUTANT == 6 ? (j) : (P) - (k) + (j))         /* replaced rank */
+
 typedef struct {
     or_task t;          /* pointers below are owned copies */
     float* rot;         /* Q[j][d] */
@@ -332,9 +347,14 @@ static void island_exchange(or_engine* e, int ti, int gen) {
             s->tfit[dst] = s->fit[src];
         }
     }
+    /* one island offset per slot k, shared by every island (the reference's
+       select_interval[iter * IPB + k] has no block term, SG-EMTF.cu:1227, :1597): slot k of
+       island q is refilled from slot k of island (q + off_k) mod n, a rotation, so the
+       exchange permutes the task's rows */
     for (int q = 0; q < n; ++q) {
         for (int k = 0; k < m; ++k) {
-            uint32_t w = draw_word(e->p.seed, (uint32_t)ti, (uint32_t)gen, OR_P_ISLX, (uint32_t)q, 1u, (uint32_t)k);
+            uint32_t w = draw_word(e->p.seed, (uint32_t)ti, (uint32_t)gen, OR_P_ISLX, MUT_XCH_INDEX(q), 1u,
+                                   (uint32_t)k);
             int src_isl = (q + (int)(w % (uint32_t)n)) % n;
             int src = src_isl * m + k, dst = q * m + k;
             memcpy(s->X + (size_t)dst * D, s->trial + (size_t)src * D, sizeof(float) * (size_t)D);
@@ -352,9 +372,9 @@ static void island_migration(or_engine* e, int ti) {
     for (int q = 0; q < n; ++q) {
         const int pdst = (q + 1) % n;
         for (int j = 0; j < k; ++j) {
-            int src = q * m + s->order[q * m + j];
-            int dst = pdst * m + s->order[pdst * m + m - k + j];
-            if (s->snapF[src] < s->snapF[dst]) {
+            int src = q * m + s->order[q * m + MUT_MIG_SRC(m, j)];
+            int dst = pdst * m + s->order[pdst * m + MUT_MIG_DST(m, k, j)];
+            if (MUT_MIG_BETTER(s->snapF[src], s->snapF[dst])) {
                 memcpy(s->X + (size_t)dst * D, s->snapX + (size_t)src * D, sizeof(float) * (size_t)D);
                 s->fit[dst] = s->snapF[src];
             }
@@ -460,8 +480,8 @@ static void transfer_task(or_engine* e, int ti, int gen, float* work) {
                     if (same_box) g = v;
                     else {
                         float t1 = v - ds->t.lo;
-                        float t2 = t1 / rng_d;
-                        float t3 = t2 * rng_s;
+                        float t2 = t1 / MUT_XFER_RD(rng_d, rng_s);
+                        float t3 = t2 * MUT_XFER_RS(rng_d, rng_s);
                         g = s->t.lo + t3;
                     }
                 }
@@ -471,7 +491,7 @@ static void transfer_task(or_engine* e, int ti, int gen, float* work) {
     }
     eval_points(s, k, cand, s->tfit, work);
     for (int j = 0; j < k; ++j) {
-        int tgt = s->order[P - k + j];
+        int tgt = s->order[MUT_XFER_TGT(P, k, j)];
         if (s->tfit[j] < s->fit[tgt]) {
             s->fit[tgt] = s->tfit[j];
             memcpy(s->X + (size_t)tgt * D, cand + (size_t)j * D, sizeof(float) * (size_t)D);
@@ -571,6 +591,25 @@ void or_run(or_engine* e, int32_t n_gens, int32_t n_threads) {
 }
 
 int32_t or_generation(const or_engine* e) { return e->gen; }
+
+/* Single phases on the current state, for the tests that pin them one at a time. */
+void or_phase_exchange(or_engine* e, int32_t task, int32_t gen) {
+    if (e->p.n_islands > 1) island_exchange(e, task, gen);
+}
+
+void or_phase_migrate(or_engine* e, int32_t task) {
+    if (e->p.n_islands > 1 && e->p.migrate_count > 0) island_migration(e, task);
+}
+
+void or_phase_transfer(or_engine* e, int32_t gen) {
+    if (e->T < 2 || e->p.transfer_count < 1) return;
+    for (int ti = 0; ti < e->T; ++ti) snapshot_elites(e, ti);
+    for (int ti = 0; ti < e->T; ++ti) {
+        float* work = (float*)malloc(sizeof(float) * 2 * (size_t)e->ts[ti].t.dim);
+        transfer_task(e, ti, gen, work);
+        free(work);
+    }
+}
 
 void or_get_population(const or_engine* e, int32_t task, float* x, float* fit) {
     const task_state* s = &e->ts[task];

@@ -3,6 +3,7 @@
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by the smoke check and by the
 CPU-baseline leg of bench.py; the product never imports it.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -52,50 +53,78 @@ def build(force=False):
 _lib = None
 
 
+def build_mutant(k, out):
+    """Builds the oracle with planted mistake OR_MUTANT=k into `out` (mutation tests only)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "mutant", f"K={int(k)}", f"OUT={out}"], check=True)
+    return out
+
+
+@contextlib.contextmanager
+def swapped(path):
+    """Routes every call made inside the block through another build of the oracle (a
+    mutant): objects created inside the block keep using it."""
+    global _lib
+    old = _lib
+    _lib = _bind(path)
+    try:
+        yield _lib
+    finally:
+        _lib = old
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        L = ctypes.CDLL(_LIB)
-        f32p = ctypes.POINTER(ctypes.c_float)
-        f64p = ctypes.POINTER(ctypes.c_double)
-        u32p = ctypes.POINTER(ctypes.c_uint32)
-        L.or_philox.argtypes = [u32p, u32p, u32p]
-        L.or_draw_block.argtypes = [ctypes.c_uint64] + [ctypes.c_uint32] * 6 + [u32p]
-        for n in ["or_sinpi_sq", "or_sin2pi", "or_expm1", "or_exp2", "or_log2", "or_weier_term"]:
-            getattr(L, n).argtypes = [ctypes.c_float]
-            getattr(L, n).restype = ctypes.c_float
-        L.or_pow.argtypes = [ctypes.c_float, ctypes.c_float]
-        L.or_pow.restype = ctypes.c_float
-        L.or_func_table.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p]
-        L.or_eval32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
-        L.or_eval64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
-        L.or_grad64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
-        L.or_grad32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
-        L.or_create.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, ctypes.POINTER(OrParams)]
-        L.or_create.restype = ctypes.c_void_p
-        L.or_destroy.argtypes = [ctypes.c_void_p]
-        L.or_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
-        L.or_generation.argtypes = [ctypes.c_void_p]
-        L.or_generation.restype = ctypes.c_int32
-        L.or_get_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
-        L.or_set_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
-        L.or_get_best.argtypes = [ctypes.c_void_p, f32p, f32p, ctypes.c_int32]
-        u8p = ctypes.POINTER(ctypes.c_uint8)
-        i32p = ctypes.POINTER(ctypes.c_int32)
-        L.or_cvrp_dist.argtypes = [ctypes.c_int32] * 4
-        L.or_cvrp_dist.restype = ctypes.c_int32
-        L.or_cvrp_split.argtypes = [u8p, ctypes.c_int, u8p, u8p, ctypes.c_int]
-        L.or_cvrp_split.restype = ctypes.c_int32
-        L.or_cvrp_child.argtypes = [u8p, u8p] + [ctypes.c_int] * 6 + [u8p]
-        L.or_cvrp_create.argtypes = [ctypes.POINTER(CvrpInstance), ctypes.c_int32, ctypes.POINTER(CvrpParams)]
-        L.or_cvrp_create.restype = ctypes.c_void_p
-        L.or_cvrp_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
-        L.or_cvrp_get.argtypes = [ctypes.c_void_p, ctypes.c_int32, u8p, i32p]
-        L.or_cvrp_destroy.argtypes = [ctypes.c_void_p]
-        _lib = L
+        _lib = _bind(_LIB)
     return _lib
+
+
+def _bind(path):
+    L = ctypes.CDLL(path)
+    f32p = ctypes.POINTER(ctypes.c_float)
+    f64p = ctypes.POINTER(ctypes.c_double)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.or_philox.argtypes = [u32p, u32p, u32p]
+    L.or_draw_block.argtypes = [ctypes.c_uint64] + [ctypes.c_uint32] * 6 + [u32p]
+    for n in ["or_sinpi_sq", "or_sin2pi", "or_expm1", "or_exp2", "or_log2", "or_weier_term"]:
+        getattr(L, n).argtypes = [ctypes.c_float]
+        getattr(L, n).restype = ctypes.c_float
+    L.or_pow.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.or_pow.restype = ctypes.c_float
+    L.or_func_table.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p]
+    L.or_eval32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
+    L.or_eval64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
+    L.or_grad64.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f64p, f64p]
+    L.or_grad32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
+    L.or_create.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, ctypes.POINTER(OrParams)]
+    L.or_create.restype = ctypes.c_void_p
+    L.or_destroy.argtypes = [ctypes.c_void_p]
+    L.or_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+    L.or_generation.argtypes = [ctypes.c_void_p]
+    L.or_generation.restype = ctypes.c_int32
+    L.or_get_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
+    L.or_set_population.argtypes = [ctypes.c_void_p, ctypes.c_int32, f32p, f32p]
+    L.or_get_best.argtypes = [ctypes.c_void_p, f32p, f32p, ctypes.c_int32]
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.or_cvrp_dist.argtypes = [ctypes.c_int32] * 4
+    L.or_cvrp_dist.restype = ctypes.c_int32
+    L.or_cvrp_split.argtypes = [u8p, ctypes.c_int, u8p, u8p, ctypes.c_int]
+    L.or_cvrp_split.restype = ctypes.c_int32
+    L.or_cvrp_child.argtypes = [u8p, u8p] + [ctypes.c_int] * 6 + [u8p]
+    L.or_cvrp_create.argtypes = [ctypes.POINTER(CvrpInstance), ctypes.c_int32, ctypes.POINTER(CvrpParams)]
+    L.or_cvrp_create.restype = ctypes.c_void_p
+    L.or_cvrp_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+    L.or_cvrp_get.argtypes = [ctypes.c_void_p, ctypes.c_int32, u8p, i32p]
+    L.or_cvrp_destroy.argtypes = [ctypes.c_void_p]
+    L.or_phase_exchange.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+    L.or_phase_migrate.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.or_phase_transfer.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.or_cvrp_set.argtypes = [ctypes.c_void_p, ctypes.c_int32, u8p, i32p]
+    L.or_cvrp_phase_migrate.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    return L
 
 
 def _fp(a):
@@ -196,38 +225,49 @@ class Engine:
         self._holders = [_TaskHolder(t) for t in tasks]
         arr = (OrTask * len(tasks))(*[h.c for h in self._holders])
         self._p = make_params(**params)
-        self._h = lib().or_create(arr, len(tasks), ctypes.byref(self._p))
+        self._L = lib()
+        self._h = self._L.or_create(arr, len(tasks), ctypes.byref(self._p))
         self.n_tasks = len(tasks)
 
     def run(self, gens, threads=0):
-        lib().or_run(self._h, int(gens), int(threads))
+        self._L.or_run(self._h, int(gens), int(threads))
 
     @property
     def generation(self):
-        return lib().or_generation(self._h)
+        return self._L.or_generation(self._h)
 
     def population(self, t):
         h = self._holders[t]
         x = np.zeros((h.pop, h.dim), np.float32)
         f = np.zeros(h.pop, np.float32)
-        lib().or_get_population(self._h, t, _fp(x), _fp(f))
+        self._L.or_get_population(self._h, t, _fp(x), _fp(f))
         return x, f
 
     def set_population(self, t, x, f):
         x = np.ascontiguousarray(x, np.float32)
         f = np.ascontiguousarray(f, np.float32)
-        lib().or_set_population(self._h, t, _fp(x), _fp(f))
+        self._L.or_set_population(self._h, t, _fp(x), _fp(f))
 
     def best(self):
         dmax = max(h.dim for h in self._holders)
         f = np.zeros(self.n_tasks, np.float32)
         x = np.zeros((self.n_tasks, dmax), np.float32)
-        lib().or_get_best(self._h, _fp(f), _fp(x), dmax)
+        self._L.or_get_best(self._h, _fp(f), _fp(x), dmax)
         return f, x
+
+    # single phases on the current state (tests that pin them one at a time)
+    def phase_exchange(self, t, gen):
+        self._L.or_phase_exchange(self._h, int(t), int(gen))
+
+    def phase_migrate(self, t):
+        self._L.or_phase_migrate(self._h, int(t))
+
+    def phase_transfer(self, gen):
+        self._L.or_phase_transfer(self._h, int(gen))
 
     def close(self):
         if self._h:
-            lib().or_destroy(self._h)
+            self._L.or_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -282,22 +322,31 @@ class CvrpEngine:
             arr.append(CvrpInstance(int(ins["n"]), int(ins["capacity"]), int(ins["pop"]), _i32(xy), _i32(dem)))
         carr = (CvrpInstance * len(arr))(*arr)
         self._p = CvrpParams(p_swap, p_2opt, n_islands, migrate_interval, migrate_count, seed)
-        self._h = lib().or_cvrp_create(carr, len(arr), ctypes.byref(self._p))
+        self._L = lib()
+        self._h = self._L.or_cvrp_create(carr, len(arr), ctypes.byref(self._p))
         self.instances = instances
 
     def run(self, gens, threads=0):
-        lib().or_cvrp_run(self._h, int(gens), int(threads))
+        self._L.or_cvrp_run(self._h, int(gens), int(threads))
 
     def population(self, i):
         ins = self.instances[i]
         t = np.zeros((ins["pop"], ins["n"]), np.uint8)
         c = np.zeros(ins["pop"], np.int32)
-        lib().or_cvrp_get(self._h, i, _u8(t), _i32(c))
+        self._L.or_cvrp_get(self._h, i, _u8(t), _i32(c))
         return t, c
+
+    def set_population(self, i, tours, costs):
+        t = np.ascontiguousarray(tours, np.uint8)
+        c = np.ascontiguousarray(costs, np.int32)
+        self._L.or_cvrp_set(self._h, i, _u8(t), _i32(c))
+
+    def phase_migrate(self, i):
+        self._L.or_cvrp_phase_migrate(self._h, int(i))
 
     def close(self):
         if self._h:
-            lib().or_cvrp_destroy(self._h)
+            self._L.or_cvrp_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -54,6 +54,9 @@ EMTF_DECL_QPAIR(2) EMTF_DECL_QPAIR(4) EMTF_DECL_QPAIR(5) EMTF_DECL_QPAIR(6)
     extern "C" int emtf_launch_polish_kb##kb(const emtf::EpochArgs* args, const int* order, int n, int lds_bytes, \
                                              hipStream_t stream);
 EMTF_DECL_OBJ(2) EMTF_DECL_OBJ(4) EMTF_DECL_OBJ(7) EMTF_DECL_OBJ(8) EMTF_DECL_OBJ(12) EMTF_DECL_OBJ(13) EMTF_DECL_OBJ(16)
+// forward objective for 128 < dim <= 32 NT (Q and points read from global memory)
+#define EMTF_DECL_OBJW(nt) extern "C" int emtf_launch_objective_nt##nt##_fwd(const emtf::ObjArgs* a, hipStream_t s);
+EMTF_DECL_OBJW(5) EMTF_DECL_OBJW(6) EMTF_DECL_OBJW(7) EMTF_DECL_OBJW(8)
 
 typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
 extern "C" int emtf_launch_trace(const emtf::TaskDev* tasks, int n_tasks, const float* fit, float* out_best,
@@ -456,6 +459,17 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         if (d.xform < EMTF_SHIFT || d.xform > EMTF_DIAG) { snprintf(buf, sizeof buf, "task %d: unknown xform", t); return fail(EMTF_E_INVALID, buf); }
         if (d.xform == EMTF_ROTATE && !d.rot) { snprintf(buf, sizeof buf, "task %d: rotation matrix missing", t); return fail(EMTF_E_INVALID, buf); }
         if (d.xform == EMTF_DIAG && !d.diag) { snprintf(buf, sizeof buf, "task %d: diagonal missing", t); return fail(EMTF_E_INVALID, buf); }
+        // a NaN or infinity in the transform or the shift would make every fitness NaN
+        auto finite = [](const float* v, size_t n) {
+            for (size_t i = 0; v && i < n; ++i)
+                if (!std::isfinite(v[i])) return false;
+            return true;
+        };
+        if ((d.xform == EMTF_ROTATE && !finite(d.rot, (size_t)d.dim * d.dim)) ||
+            (d.xform == EMTF_DIAG && !finite(d.diag, (size_t)d.dim)) || !finite(d.shift, (size_t)d.dim)) {
+            snprintf(buf, sizeof buf, "task %d: non-finite value in rot / diag / shift", t);
+            return fail(EMTF_E_INVALID, buf);
+        }
         const int m = d.pop / nis;
         if (prm->algo == EMTF_GA_SBX_PM && (m % 2)) { snprintf(buf, sizeof buf, "task %d: GA needs an even island size", t); return fail(EMTF_E_INVALID, buf); }
         if (prm->transfer_interval > 0 && 2 * prm->transfer_count > d.pop) { snprintf(buf, sizeof buf, "task %d: transfer_count > pop/2", t); return fail(EMTF_E_INVALID, buf); }
@@ -929,7 +943,8 @@ static int objective_call(int32_t func, int32_t dim, int32_t n, const float* x, 
     g_err.clear();
     if (func < EMTF_SPHERE || func > EMTF_ELLIPTIC) return fail(EMTF_E_INVALID, "unknown func");
     if (dim < 1) return fail(EMTF_E_INVALID, "dim < 1");
-    if (dim > 128) return fail(EMTF_E_UNSUPPORTED, "dim > 128 not built");
+    if (dim > 256) return fail(EMTF_E_UNSUPPORTED, "dim > 256 not built");
+    if (adjoint && dim > 128) return fail(EMTF_E_UNSUPPORTED, "adjoint: dim > 128 not built");
     if (n < 0 || !x || (adjoint ? (!gf || !gx) : !f)) return fail(EMTF_E_INVALID, "NULL buffer or n < 0");
     if (n == 0) return EMTF_OK;
     emtf::ObjArgs a{};
@@ -945,14 +960,23 @@ static int objective_call(int32_t func, int32_t dim, int32_t n, const float* x, 
     a.gx = gx;
     func_table(func, dim, a.tab);
     int rc = -1;
-    switch (pick_kb(dim)) {
-    case 2: rc = emtf_launch_objective_kb2(&a, adjoint, (hipStream_t)stream); break;
-    case 4: rc = emtf_launch_objective_kb4(&a, adjoint, (hipStream_t)stream); break;
-    case 7: rc = emtf_launch_objective_kb7(&a, adjoint, (hipStream_t)stream); break;
-    case 8: rc = emtf_launch_objective_kb8(&a, adjoint, (hipStream_t)stream); break;
-    case 12: rc = emtf_launch_objective_kb12(&a, adjoint, (hipStream_t)stream); break;
-    case 13: rc = emtf_launch_objective_kb13(&a, adjoint, (hipStream_t)stream); break;
-    case 16: rc = emtf_launch_objective_kb16(&a, adjoint, (hipStream_t)stream); break;
+    if (dim > 128) {
+        switch (pick_nt(dim)) {
+        case 5: rc = emtf_launch_objective_nt5_fwd(&a, (hipStream_t)stream); break;
+        case 6: rc = emtf_launch_objective_nt6_fwd(&a, (hipStream_t)stream); break;
+        case 7: rc = emtf_launch_objective_nt7_fwd(&a, (hipStream_t)stream); break;
+        case 8: rc = emtf_launch_objective_nt8_fwd(&a, (hipStream_t)stream); break;
+        }
+    } else {
+        switch (pick_kb(dim)) {
+        case 2: rc = emtf_launch_objective_kb2(&a, adjoint, (hipStream_t)stream); break;
+        case 4: rc = emtf_launch_objective_kb4(&a, adjoint, (hipStream_t)stream); break;
+        case 7: rc = emtf_launch_objective_kb7(&a, adjoint, (hipStream_t)stream); break;
+        case 8: rc = emtf_launch_objective_kb8(&a, adjoint, (hipStream_t)stream); break;
+        case 12: rc = emtf_launch_objective_kb12(&a, adjoint, (hipStream_t)stream); break;
+        case 13: rc = emtf_launch_objective_kb13(&a, adjoint, (hipStream_t)stream); break;
+        case 16: rc = emtf_launch_objective_kb16(&a, adjoint, (hipStream_t)stream); break;
+        }
     }
     if (rc != 0) return fail(EMTF_E_DEVICE, std::string("objective kernel launch: ") + hipGetErrorString((hipError_t)rc));
     return EMTF_OK;

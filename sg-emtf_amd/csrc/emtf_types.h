@@ -64,7 +64,7 @@ struct ObjArgs {
     const float* gf;     // [n] upstream gradient (adjoint)
     float* f;            // [n] objective (forward)
     float* gx;           // [n][dim] gradient (adjoint)
-    float tab[128];      // per-dimension table (Elliptic weights / Griewank scales)
+    float tab[256];      // per-dimension table (Elliptic weights / Griewank scales)
 };
 
 }  // namespace emtf

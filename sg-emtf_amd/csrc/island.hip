@@ -613,9 +613,12 @@ __device__ __forceinline__ void island_exchange(const EpochArgs& a, const Geo& g
         for (int p = 0; p < m; ++p) L.ord[q * m + arr[p]] = p;  // inverse permutation
     }
     __syncthreads();
+    // one island offset per slot k, shared by every island (the reference's
+    // select_interval[iter * IPB + k] has no block term, SG-EMTF.cu:1227, :1597): each slot
+    // is a rotation of the islands, so the exchange permutes the task's rows
     for (int i = tid; i < g.pop; i += NTHR) {
         const int q = i / m, k = i - q * m;
-        const uint32_t w = rng.word((uint32_t)gen, P_ISLX, (uint32_t)q, 1u, (uint32_t)k);
+        const uint32_t w = rng.word((uint32_t)gen, P_ISLX, 0u, 1u, (uint32_t)k);
         const int src = (q + (int)(w % (uint32_t)n)) % n;
         L.aux[i] = src * m + L.ord[src * m + k];
     }
@@ -1242,7 +1245,7 @@ __global__ __launch_bounds__(2 * ObjShape<KB>::PB) void objective_kernel(ObjArgs
             for (int r = 0; r < 16; ++r) zrow[32 * jt + 8 * (r >> 2) + 4 * h + (r & 3)] = acc[jt][r];
         __syncthreads();
         float dz[HS];
-        dfdz<KB, FUNC>(dz, zrow, Pre + (wave * 32 + col) * ZS + (h ? 0 : 0), D, tab, h);
+        dfdz<KB, FUNC>(dz, zrow, Pre + (wave * 32 + col) * ZS, D, tab, h);
         const float up = valid ? a.gf[pt] : 0.0f;
 #pragma unroll
         for (int s = 0; s < HS; ++s) dz[s] = dz[s] * up;
@@ -1650,6 +1653,41 @@ __global__ __launch_bounds__(WT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     default: break;
     }
 }
+
+// emtf_objective_fwd for dimensions past the LDS evaluator's (D <= 32 NT): Q and the
+// points are read row-major from global memory, one gene pair per MFMA k-step in
+// increasing order with zero padding past D -- the engine evaluator's fma chains and
+// reductions, so the same bits.  128 points per workgroup, 32 per wave.
+template <int NT, int FUNC>
+__global__ __launch_bounds__(256) void objective_wide_kernel(ObjArgs a) {
+    __shared__ float o_s[32 * NT], tab_s[32 * NT];
+    const int tid = threadIdx.x, D = a.dim;
+    for (int d = tid; d < 32 * NT; d += 256) {
+        o_s[d] = (d < D && a.shift) ? a.shift[d] : 0.0f;
+        tab_s[d] = d < D ? a.tab[d] : 0.0f;
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
+    const int pt = blockIdx.x * 128 + wave * 32 + col;
+    const bool valid = pt < a.n;
+    const float* xr = a.x + (int64_t)(valid ? pt : 0) * D;
+    f16v acc[NT];
+    zero_acc<NT>(acc);
+    const int Dh = (D + 1) >> 1;
+    for (int p = 0; p < Dh; ++p) {
+        const int d = 2 * p + h;
+        const float w = d < D ? xr[d] - o_s[d] : 0.0f;
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) {
+            const int row = 32 * jt + col;
+            float q = 0.0f;
+            if (row < D && d < D) q = a.has_rot ? a.rot[(int64_t)row * D + d] : (row == d ? 1.0f : 0.0f);
+            acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(q, w, acc[jt], 0, 0, 0);
+        }
+    }
+    const float f = objective<NT, FUNC, true>(acc, D, tab_s, h);
+    if (valid && h == 0) a.f[pt] = f;
+}
 #endif  // EMTF_WIDE_NT
 
 }  // namespace emtf
@@ -1701,6 +1739,29 @@ extern "C" int EMTF_WNAME(emtf_wideq_lds_floats_nt, EMTF_WIDE_NT, a, EMTF_ALGO)(
     const int popP = (pop + 31) & ~31;
     using S = emtf::WShape<EMTF_WIDE_NT>;
     return 32 * EMTF_WIDE_NT * S::RS + S::RS + 32 * EMTF_WIDE_NT + 4 * popP;
+}
+#endif
+
+#if EMTF_ALGO == 0 && EMTF_WIDE_NT >= 5
+// emtf_objective_fwd for 128 < dim <= 32 NT (forward only)
+template <int FUNC>
+static hipError_t launch_obj_wide(const emtf::ObjArgs& a, hipStream_t stream) {
+    hipLaunchKernelGGL((emtf::objective_wide_kernel<EMTF_WIDE_NT, FUNC>), dim3((a.n + 127) / 128), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+extern "C" int EMTF_WNAME(emtf_launch_objective_nt, EMTF_WIDE_NT, fw, d)(const emtf::ObjArgs* a, hipStream_t s) {
+    switch (a->func) {
+    case 0: return (int)launch_obj_wide<0>(*a, s);
+    case 1: return (int)launch_obj_wide<1>(*a, s);
+    case 2: return (int)launch_obj_wide<2>(*a, s);
+    case 3: return (int)launch_obj_wide<3>(*a, s);
+    case 4: return (int)launch_obj_wide<4>(*a, s);
+    case 5: return (int)launch_obj_wide<5>(*a, s);
+    case 6: return (int)launch_obj_wide<6>(*a, s);
+    case 7: return (int)launch_obj_wide<7>(*a, s);
+    }
+    return (int)hipErrorInvalidValue;
 }
 #endif
 #else

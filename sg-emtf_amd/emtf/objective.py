@@ -32,10 +32,25 @@ def _check_input(x):
         raise TypeError("x must be a float32 CUDA tensor of shape [n, dim]")
 
 
+def _operand(t, x, shape, name):
+    """rot [dim, dim] / shift [dim] as a contiguous float32 tensor on x's device: the kernels
+    read them as row-major binary32, so any other layout, dtype or device is refused."""
+    import torch
+    if t is None:
+        return None
+    if not (isinstance(t, torch.Tensor) and t.dtype == torch.float32 and t.device == x.device):
+        raise TypeError(f"{name} must be a float32 tensor on {x.device}")
+    if tuple(t.shape) != shape:
+        raise ValueError(f"{name} must have shape {list(shape)}, got {list(t.shape)}")
+    return t.contiguous()
+
+
 def forward_raw(func, x, rot=None, shift=None):
     import torch
     _check_input(x)
     x = x.contiguous()
+    D = x.shape[1]
+    rot, shift = _operand(rot, x, (D, D), "rot"), _operand(shift, x, (D,), "shift")
     f = torch.empty(x.shape[0], device=x.device, dtype=torch.float32)
     stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
     _lib.check(_bind().emtf_objective_fwd(int(func), x.shape[1], x.shape[0], _ptr(x), _ptr(rot), _ptr(shift),
@@ -47,6 +62,8 @@ def adjoint_raw(func, x, gf, rot=None, shift=None):
     import torch
     _check_input(x)
     x = x.contiguous()
+    D = x.shape[1]
+    rot, shift = _operand(rot, x, (D, D), "rot"), _operand(shift, x, (D,), "shift")
     gf = gf.contiguous().to(torch.float32)
     gx = torch.empty_like(x)
     stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)

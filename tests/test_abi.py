@@ -72,6 +72,24 @@ def test_invalid_task_descriptors(kw, expect, msg):
     assert rc == expect and msg in err, (rc, err)
 
 
+@pytest.mark.parametrize("where", ["rot", "diag", "shift"])
+@pytest.mark.parametrize("bad", [float("nan"), float("inf"), -float("inf")])
+def test_non_finite_task_arrays_rejected(where, bad):
+    """A NaN or infinity in the rotation, the diagonal or the shift is refused at create."""
+    from emtf import _lib
+    D = 6
+    rot = np.eye(D, dtype=np.float32)
+    diag = np.ones(D, np.float32)
+    shift = np.zeros(D, np.float32)
+    {"rot": rot, "diag": diag, "shift": shift}[where].ravel()[D + 1 if where == "rot" else 3] = bad
+    f32p = _lib.f32p
+    xform = {"rot": 1, "diag": 2, "shift": 1}[where]
+    d = _lib.TaskDesc(0, D, 32, xform, -1.0, 1.0, rot.ctypes.data_as(f32p) if xform == 1 else None,
+                      shift.ctypes.data_as(f32p), diag.ctypes.data_as(f32p) if xform == 2 else None)
+    rc, err = _create([d])
+    assert rc == -1 and "non-finite" in err, (rc, err)
+
+
 def test_invalid_params():
     d, keep = _desc()
     assert _create([d], algo=7)[0] == -1

@@ -135,6 +135,21 @@ def test_gpu_forward_bit_exact(oracle_mod, D):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("D", [129, 150, 200, 256])
+def test_gpu_forward_wide_dims_bit_exact(oracle_mod, D):
+    """Dimensions past the LDS evaluator (Q and points read from global memory)."""
+    from emtf.objective import forward_raw
+    for func in range(8):
+        for rotate in (False, True):
+            t = _task(func, D, rotate, 700 + func)
+            x = _points(t, 150, func, 20.0)
+            print("case: wide fwd D", D, "func", func, "rotate", rotate, flush=True)
+            f = forward_raw(func, _cuda(x), _cuda(t.get("rot")), _cuda(t["shift"])).cpu().numpy()
+            ref = oracle_mod.eval32(t, x)
+            assert np.array_equal(f.view(np.uint32), ref.view(np.uint32)), (func, rotate)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D", [7, 50, 100, 128])
 def test_gpu_adjoint_bit_exact(oracle_mod, D):
     """emtf_objective_adjoint with unit upstream gradient equals the oracle's grad32 bit for bit."""
@@ -189,3 +204,30 @@ def test_gpu_autograd_fd_check(oracle_mod):
             e[d] = h
             fd = (oracle_mod.eval64(t, xd + e) - oracle_mod.eval64(t, xd - e)) / (2 * h)
             assert np.allclose(g[:, d], fd, rtol=2e-3, atol=2e-3 * (1 + np.abs(fd).max())), (func, d)
+
+
+def test_objective_operands_are_checked():
+    """rot / shift reach the kernels as contiguous float32 on x's device, else are refused."""
+    from emtf.objective import _operand
+    x = torch.zeros(3, 4)
+    rot = torch.arange(16, dtype=torch.float32).reshape(4, 4)
+    r = _operand(rot.t(), x, (4, 4), "rot")
+    assert r.is_contiguous() and torch.equal(r, rot.t())
+    with pytest.raises(TypeError):
+        _operand(rot.double(), x, (4, 4), "rot")
+    with pytest.raises(ValueError):
+        _operand(torch.zeros(3), x, (4,), "shift")
+    assert _operand(None, x, (4,), "shift") is None
+
+
+@pytest.mark.gpu
+def test_gpu_transposed_rotation_view():
+    """A transposed view of Q gives the same f as its contiguous copy."""
+    from emtf.objective import forward_raw
+    rs = np.random.default_rng(4)
+    D = 30
+    q = torch.tensor(np.linalg.qr(rs.standard_normal((D, D)))[0], dtype=torch.float32, device="cuda:0")
+    x = torch.tensor(rs.uniform(-5, 5, (40, D)), dtype=torch.float32, device="cuda:0")
+    a = forward_raw(3, x, q.t())
+    b = forward_raw(3, x, q.t().contiguous())
+    assert torch.equal(a, b)

@@ -309,15 +309,16 @@ def test_full_size_plain_properties():
 
 def test_full_size_uneven_properties():
     """The batched uneven configuration at its real size (4096 tasks, D ~ U{10..200}, pop ~
-    U{32..256}), 200 generations: best never increases, stored fitness equals a fresh
-    evaluation (sampled tasks with D <= 128, the objective kernel's range)."""
+    U{32..256}) over all 2000 generations: best never increases, stored fitness equals a
+    fresh evaluation (every 31st task, the wide dimensions included)."""
     tasks = tg.uneven_tasks(4096, seed=0)
     gpu = emtf.Engine(tasks, seed=0, algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=20, transfer_count=4)
-    gpu.trace_start(20, 11)
-    gpu.run(200)
+    gpu.trace_start(100, 21)
+    gpu.run(2000)
     rows = gpu.trace()
+    assert rows.shape == (4096, 22)
     assert np.all(np.diff(rows[:, :-1], axis=1) <= 0)
-    ids = [t for t in range(0, 4096, 31) if tasks[t]["dim"] <= 128]
-    assert len(ids) > 50
+    ids = list(range(0, 4096, 31))
+    assert sum(tasks[t]["dim"] > 128 for t in ids) > 30
     _fitness_matches_fresh_evaluation(gpu, tasks, ids)
     gpu.close()
