@@ -73,7 +73,18 @@ typedef struct emtf_params {
                                    2: the first alternative variant where one exists (the wide
                                    kernel, or its 512-thread build); 3: the wide kernel with Q in
                                    LDS where it fits (2 and 3 for testing) */
+    int32_t formulas;           /* EMTF_FORMULAS_*: which formulas evaluate the objectives */
 } emtf_params;
+
+/* Objective formulas.  TEXTBOOK: the functions in cancellation-free binary32 forms with
+ * exact constants (zero at every optimum).  REFERENCE_DE: the reference's own formulas and
+ * constants -- evaluation() of SG-EMTF_DE.cu:834-1069: Ackley with E = 2.718282, Rastrigin
+ * as z^2 - 10 cospi(2z) + 10, Griewank's product of cosf(z / sqrtf(j+1)), Weierstrass with
+ * the binary32-rounded cospif arguments and the constant computed from PI = 3.1415926
+ * (SG-EMTF.cu:354-361), Schwefel with 418.9829.  REFERENCE_GA: the same, as the GA copy
+ * (SG-EMTF.cu:898-1111) has it: Griewank without the product (SG-EMTF.cu:1029).  Sphere,
+ * Rosenbrock and Elliptic are the same in every mode.  Gradient polish needs TEXTBOOK. */
+enum { EMTF_FORMULAS_TEXTBOOK = 0, EMTF_FORMULAS_REFERENCE_DE = 1, EMTF_FORMULAS_REFERENCE_GA = 2 };
 
 /* How emtf_run launches work.  EPOCHS (what AUTO selects): one launch per transfer
  * interval, the kernel boundary ordering the elite mailbox.  PERSISTENT (one launch

@@ -39,10 +39,16 @@
 extern "C" {
 #endif
 
+/* 0..7: the objectives in this engine's binary32 forms (cancellation-free, textbook
+ * constants).  8..13: the reference's own formulas and constants (the evaluation() of
+ * SG-EMTF_DE.cu:834-1069; 13 is the GA copy's Griewank, SG-EMTF.cu:1029, whose product is
+ * disabled); 12 and 13 share the bodies of 6 and 10, their tables differ. */
 enum {
     OR_FUNC_SPHERE = 0, OR_FUNC_ROSENBROCK = 1, OR_FUNC_ACKLEY = 2, OR_FUNC_RASTRIGIN = 3,
     OR_FUNC_GRIEWANK = 4, OR_FUNC_WEIERSTRASS = 5, OR_FUNC_SCHWEFEL = 6, OR_FUNC_ELLIPTIC = 7,
-    OR_NUM_FUNCS = 8
+    OR_FUNC_REF_ACKLEY = 8, OR_FUNC_REF_RASTRIGIN = 9, OR_FUNC_REF_GRIEWANK = 10, OR_FUNC_REF_WEIERSTRASS = 11,
+    OR_FUNC_REF_SCHWEFEL = 12, OR_FUNC_REF_GRIEWANK_GA = 13,
+    OR_NUM_FUNCS = 14
 };
 enum { OR_XFORM_SHIFT = 0, OR_XFORM_ROTATE = 1, OR_XFORM_DIAG = 2 };
 enum { OR_ALGO_DE_RAND_1_BIN = 0, OR_ALGO_DE_CURRENT_1_BIN = 1, OR_ALGO_GA_SBX_PM = 2 };
@@ -91,8 +97,14 @@ float or_exp2(float y);
 float or_log2(float x);
 float or_pow(float x, float y);   /* x >= 0 */
 float or_weier_term(float z);     /* sum_k 2^(1-k) sin^2(pi 3^k z), k = 0..20 */
+float or_cospi(float y);          /* cos(pi y), any finite y (exact reduction) */
+float or_exp(float x);            /* e^x = expm1(x) + 1 */
+float or_weier_ref_term(float z); /* sum_k 0.5^k cospi(2 (z + 0.5) 3^k) in binary32, k = 0..20 */
+float or_weier_ref_w(void);       /* the reference's Weierstrass constant (PI = 3.1415926) */
 
-/* Per-dimension coefficient tables (Elliptic weights / Griewank scales). */
+/* Per-dimension coefficient tables: Elliptic weights, Griewank scales, the Schwefel
+ * constant, the reference Griewank divisors sqrt(j+1) (+inf for the GA copy: each
+ * factor is then cos(0) = 1) and the reference Weierstrass constant. */
 void or_func_table(int32_t func, int32_t dim, float* table);
 
 /* Objective from z (contract arithmetic), table from or_func_table. */

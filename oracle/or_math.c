@@ -186,3 +186,53 @@ float or_weier_term(float z) {
     }
     return acc;
 }
+
+/* ---- the reference's own formulas (evaluation(), SG-EMTF_DE.cu:834-1069) ---------- */
+
+/* cos(pi y) = 1 - 2 sin^2(pi y / 2): y / 2 is exact and sin^2 reduces modulo 1 exactly,
+ * so every finite y works; |y| >= 2^24 (even integers) gives exactly 1, as cos(pi y) does.
+ * Stands in for CUDA's cospif (absolute error <= 2.4e-7). */
+float or_cospi(float y) {
+    float h = 0.5f * y;
+    float s = or_sinpi_sq(h);
+    float t = 2.0f * s;
+    return 1.0f - t;
+}
+
+/* e^x (stands in for CUDA's expf; relative error <= 3e-7 on [-20, 2]) */
+float or_exp(float x) { return or_expm1(x) + 1.0f; }
+
+/* sum_{k=0}^{20} 0.5^k cospi(tv 3^k), tv = 2 (z + 0.5) rounded to binary32, the powers
+ * 3^k from the binary32 recurrence p <- 3 p (equal to P3F), accumulated in increasing k
+ * in binary32: the reference's temp_val / pow_val1 / pow_val2 loop, SG-EMTF_DE.cu:1012-1022.
+ * The product tv 3^k is rounded to binary32 as the reference's is, which loses every
+ * fraction bit once it passes 2^24 (cospi of an even integer: 1). */
+float or_weier_ref_term(float z) {
+    float zh = z + 0.5f;
+    float tv = 2.0f * zh;
+    float acc = 0.0f, p1 = 1.0f;
+    for (int k = 0; k <= 20; ++k) {
+        float y = tv * P3F[k];
+        float c = or_cospi(y);
+        float t = p1 * c;
+        acc = acc + t;
+        p1 = p1 * 0.5f;
+    }
+    return acc;
+}
+
+/* Weierstrass_para (SG-EMTF.cu:354-361): res += a cos(2 PI b 0.5) over k = 0..20 in
+ * binary32 with PI = 3.1415926, the argument formed in double and rounded to binary32,
+ * a = 0.5^k, b = 3^k (binary32 recurrence); the cosine correctly rounded to binary32
+ * (the reference calls the host's cosf). */
+float or_weier_ref_w(void) {
+    float a = 1.0f, res = 0.0f;
+    for (int k = 0; k <= 20; ++k) {
+        double arg = 2.0 * 3.1415926 * (double)P3F[k] * 0.5;
+        float c = (float)cos((double)(float)arg);
+        float t = a * c;
+        res = res + t;
+        a = a * 0.5f;
+    }
+    return res;
+}

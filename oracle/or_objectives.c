@@ -27,14 +27,22 @@
 
 #define SCHWEFEL_C 418.9828872724338
 #define SCHWEFEL_CF 418.982887f
+#define SCHWEFEL_REF_CF 418.9829f   /* the reference's constant, SG-EMTF_DE.cu:1061 */
 #define E_F 2.71828175f
+#define E_REF 2.718282               /* the reference's E (a double literal), SG-EMTF_DE.cu:48 */
 
 void or_func_table(int32_t func, int32_t dim, float* table) {
+    const float w_ref = (func == OR_FUNC_REF_WEIERSTRASS) ? or_weier_ref_w() : 0.0f;
     for (int j = 0; j < dim; ++j) {
         double v = 0.0;
         if (func == OR_FUNC_ELLIPTIC) v = (dim > 1) ? pow(1.0e6, (double)j / (double)(dim - 1)) : 1.0;
         else if (func == OR_FUNC_GRIEWANK) v = 1.0 / (2.0 * M_PI * sqrt((double)(j + 1)));
         table[j] = (float)v;
+        if (func == OR_FUNC_SCHWEFEL) table[j] = SCHWEFEL_CF;
+        else if (func == OR_FUNC_REF_SCHWEFEL) table[j] = SCHWEFEL_REF_CF;
+        else if (func == OR_FUNC_REF_GRIEWANK) table[j] = sqrtf((float)(j + 1));
+        else if (func == OR_FUNC_REF_GRIEWANK_GA) table[j] = INFINITY;
+        else if (func == OR_FUNC_REF_WEIERSTRASS) table[j] = w_ref;
     }
 }
 
@@ -127,7 +135,8 @@ float or_finish32(const or_task* t, const float* z, const float* tab) {
     case OR_FUNC_WEIERSTRASS:
         for (int j = 0; j < D; ++j) { float v = or_weier_term(z[j]); if (ACC_A(j)) A = A + v; else B = B + v; }
         return A + B;
-    case OR_FUNC_SCHWEFEL: {
+    case OR_FUNC_SCHWEFEL:
+    case OR_FUNC_REF_SCHWEFEL: { /* the constant (textbook or the reference's) from the table */
         for (int j = 0; j < D; ++j) {
             float sq = sqrtf(fabsf(z[j]));
             float tv = sq * 0.159154937f;
@@ -135,9 +144,63 @@ float or_finish32(const or_task* t, const float* z, const float* tab) {
             float v = z[j] * s;
             if (ACC_A(j)) A = A + v; else B = B + v;
         }
-        float cd = SCHWEFEL_CF * (float)D;
+        float cd = tab[0] * (float)D;
         float S = A + B;
         return cd - S;
+    }
+    /* ---- the reference's formulas, SG-EMTF_DE.cu:834-1069, in binary32 ---- */
+    case OR_FUNC_REF_ACKLEY: { /* -20 expf(-0.2 sqrtf(S/D)) - expf(C/D) + 20 + E, C = sum cospi(2 z) */
+        for (int j = 0; j < D; ++j) {
+            float v = z[j] * z[j];
+            float tz = 2.0f * z[j];
+            float c = or_cospi(tz);
+            if (ACC_A(j)) { A = A + v; A2 = A2 + c; } else { B = B + v; B2 = B2 + c; }
+        }
+        float Df = (float)D;
+        float S = A + B, C = A2 + B2;
+        float q = S / Df;
+        float r = sqrtf(q);
+        float a = (float)(-0.2 * (double)r); /* -0.2 is a double literal in the reference */
+        float e1 = or_exp(a);
+        float t1 = -20.0f * e1;
+        float c2 = C / Df;
+        float e2 = or_exp(c2);
+        float u = t1 - e2;
+        u = u + 20.0f;
+        return (float)((double)u + E_REF);
+    }
+    case OR_FUNC_REF_RASTRIGIN: /* z^2 - 10 cospi(2 z) + 10 per dimension */
+        for (int j = 0; j < D; ++j) {
+            float zz = z[j] * z[j];
+            float tz = 2.0f * z[j];
+            float c = or_cospi(tz);
+            float t = 10.0f * c;
+            float v = zz - t;
+            v = v + 10.0f;
+            if (ACC_A(j)) A = A + v; else B = B + v;
+        }
+        return A + B;
+    case OR_FUNC_REF_GRIEWANK:
+    case OR_FUNC_REF_GRIEWANK_GA: { /* 1 + S / 4000 - prod cosf(z_j / sqrtf(j + 1)) */
+        float pA = 1.0f, pB = 1.0f;
+        for (int j = 0; j < D; ++j) {
+            float v = z[j] * z[j];
+            float q = z[j] / tab[j];
+            float tq = q * 0.159154937f;
+            float c = 1.0f - 2.0f * or_sinpi_sq(tq); /* cos(q) = 1 - 2 sin^2(q / 2) */
+            if (ACC_A(j)) { A = A + v; pA = pA * c; } else { B = B + v; pB = pB * c; }
+        }
+        float P = pA * pB;
+        float S = A + B;
+        float s4 = S / 4000.0f;
+        float u = 1.0f + s4;
+        return u - P;
+    }
+    case OR_FUNC_REF_WEIERSTRASS: { /* sum of terms - D W_ref */
+        for (int j = 0; j < D; ++j) { float v = or_weier_ref_term(z[j]); if (ACC_A(j)) A = A + v; else B = B + v; }
+        float S = A + B;
+        float dw = (float)D * tab[0];
+        return S - dw;
     }
     case OR_FUNC_ELLIPTIC:
         for (int j = 0; j < D; ++j) {

@@ -14,6 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
 FUNC_NAMES = ["sphere", "rosenbrock", "ackley", "rastrigin", "griewank", "weierstrass", "schwefel", "elliptic"]
+# the reference's own formulas (or_objectives.c): ids 8..13
+REF_FUNC_IDS = {"ackley": 8, "rastrigin": 9, "griewank": 10, "weierstrass": 11, "schwefel": 12, "griewank_ga": 13}
 XFORM_SHIFT, XFORM_ROTATE, XFORM_DIAG = 0, 1, 2
 ALGO_DE_RAND_1_BIN, ALGO_DE_CURRENT_1_BIN, ALGO_GA_SBX_PM = 0, 1, 2
 
@@ -88,9 +90,12 @@ def _bind(path):
     u32p = ctypes.POINTER(ctypes.c_uint32)
     L.or_philox.argtypes = [u32p, u32p, u32p]
     L.or_draw_block.argtypes = [ctypes.c_uint64] + [ctypes.c_uint32] * 6 + [u32p]
-    for n in ["or_sinpi_sq", "or_sin2pi", "or_expm1", "or_exp2", "or_log2", "or_weier_term"]:
+    for n in ["or_sinpi_sq", "or_sin2pi", "or_expm1", "or_exp2", "or_log2", "or_weier_term", "or_cospi", "or_exp",
+              "or_weier_ref_term"]:
         getattr(L, n).argtypes = [ctypes.c_float]
         getattr(L, n).restype = ctypes.c_float
+    L.or_weier_ref_w.argtypes = []
+    L.or_weier_ref_w.restype = ctypes.c_float
     L.or_pow.argtypes = [ctypes.c_float, ctypes.c_float]
     L.or_pow.restype = ctypes.c_float
     L.or_func_table.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p]

@@ -145,7 +145,7 @@ static launch_fn pick_polish(int kb, polish_lds_fn* lds) {
 // dispatch the expensive tasks first.
 static double func_cost(int f) {
     switch (f) {
-    case EMTF_WEIERSTRASS: return 21.0;
+    case EMTF_WEIERSTRASS: case emtf::F_REF_WEIERSTRASS: return 21.0;
     case EMTF_SCHWEFEL: case EMTF_ACKLEY: case EMTF_RASTRIGIN: case EMTF_GRIEWANK: return 2.0;
     default: return 1.0;
     }
@@ -265,13 +265,53 @@ static int use_stream(emtf_handle* h, hipStream_t s) {
     return EMTF_OK;
 }
 
-// Per-dimension table: Elliptic weights 10^(6 j/(D-1)), Griewank 1/(2 pi sqrt(j+1)).
+// The reference's Weierstrass constant Weierstrass_para (SG-EMTF.cu:354-361): sum of
+// 0.5^k cos(2 PI 3^k 0.5) over k = 0..20 in binary32, PI = 3.1415926, the argument formed
+// in double and rounded to binary32, its cosine correctly rounded to binary32.
+static float weierstrass_para_ref() {
+    static const float p3[21] = {1.f, 3.f, 9.f, 27.f, 81.f, 243.f, 729.f, 2187.f, 6561.f, 19683.f,
+                                 59049.f, 177147.f, 531441.f, 1594323.f, 4782969.f, 14348907.f,
+                                 43046721.f, 129140163.f, 387420489.f, 1162261467.f, 3486784401.f};
+    float a = 1.0f, res = 0.0f;
+    for (int k = 0; k <= 20; ++k) {
+        const double arg = 2.0 * 3.1415926 * (double)p3[k] * 0.5;
+        const float c = (float)std::cos((double)(float)arg);
+        const float t = a * c;
+        res = res + t;
+        a = a * 0.5f;
+    }
+    return res;
+}
+
+// Per-dimension table: Elliptic weights 10^(6 j/(D-1)), Griewank 1/(2 pi sqrt(j+1)), the
+// Schwefel constant (textbook 418.982887, reference 418.9829); reference forms: Griewank
+// divisors sqrtf(j+1) (+inf for the GA copy, whose product is disabled) and the
+// Weierstrass constant.
 static void func_table(int func, int dim, float* t) {
+    const float w_ref = func == emtf::F_REF_WEIERSTRASS ? weierstrass_para_ref() : 0.0f;
     for (int j = 0; j < dim; ++j) {
         double v = 0.0;
         if (func == EMTF_ELLIPTIC) v = (dim > 1) ? std::pow(1.0e6, (double)j / (double)(dim - 1)) : 1.0;
         else if (func == EMTF_GRIEWANK) v = 1.0 / (2.0 * M_PI * std::sqrt((double)(j + 1)));
         t[j] = (float)v;
+        if (func == EMTF_SCHWEFEL) t[j] = 418.982887f;
+        else if (func == emtf::F_REF_SCHWEFEL) t[j] = 418.9829f;
+        else if (func == emtf::F_REF_GRIEWANK) t[j] = std::sqrt((float)(j + 1));
+        else if (func == emtf::F_REF_GRIEWANK_GA) t[j] = INFINITY;
+        else if (func == emtf::F_REF_WEIERSTRASS) t[j] = w_ref;
+    }
+}
+
+// The objective id the kernels run for a public id under the chosen formulas.
+static int kernel_func(int func, int formulas) {
+    if (formulas == EMTF_FORMULAS_TEXTBOOK) return func;
+    switch (func) {
+    case EMTF_ACKLEY: return emtf::F_REF_ACKLEY;
+    case EMTF_RASTRIGIN: return emtf::F_REF_RASTRIGIN;
+    case EMTF_GRIEWANK: return formulas == EMTF_FORMULAS_REFERENCE_GA ? emtf::F_REF_GRIEWANK_GA : emtf::F_REF_GRIEWANK;
+    case EMTF_WEIERSTRASS: return emtf::F_REF_WEIERSTRASS;
+    case EMTF_SCHWEFEL: return emtf::F_REF_SCHWEFEL;
+    default: return func;
     }
 }
 
@@ -445,6 +485,11 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         return fail(EMTF_E_INVALID, "unknown schedule");
     if (prm->tune < 0 || prm->tune > 3)
         return fail(EMTF_E_INVALID, "tune must be 0 (calibrate), 1 (off), 2 or 3 (one alternative)");
+    if (prm->formulas < EMTF_FORMULAS_TEXTBOOK || prm->formulas > EMTF_FORMULAS_REFERENCE_GA)
+        return fail(EMTF_E_INVALID, "unknown formulas");
+    if (prm->formulas != EMTF_FORMULAS_TEXTBOOK && prm->polish_interval > 0 && prm->polish_steps > 0 &&
+        prm->polish_topk > 0)
+        return fail(EMTF_E_UNSUPPORTED, "gradient polish needs the textbook formulas");
 
     int max_dim = 0;
     for (int t = 0; t < n_tasks; ++t) {
@@ -625,7 +670,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         const Bucket& b = h->buckets[bucket_of[t]];
         emtf::TaskDev& td = h->tasks[t];
         std::memset(&td, 0, sizeof td);
-        td.func = d.func;
+        td.func = kernel_func(d.func, prm->formulas);
         td.dim = d.dim;
         td.pop = d.pop;
         td.xform = d.xform;
@@ -766,7 +811,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
             else q[(size_t)j * d.dim + j] = (d.xform == EMTF_DIAG) ? d.diag[j] : 1.0f;
         }
         if (d.shift) std::memcpy(V.data() + td.v_off, d.shift, sizeof(float) * d.dim);
-        func_table(d.func, d.dim, TB.data() + td.v_off);
+        func_table(td.func, d.dim, TB.data() + td.v_off);
     }
     // wide tasks: Q rows in the kernels' [even | odd] row layout, zero padded to 32 NT rows
     std::vector<float> QW((size_t)nW, 0.0f);

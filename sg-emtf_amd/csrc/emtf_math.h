@@ -229,6 +229,39 @@ __device__ __forceinline__ float weier_term(float z) {
     return acc;
 }
 
+// ---- the reference's own formulas (evaluation(), SG-EMTF_DE.cu:834-1069) ----------
+// cos(pi y) = 1 - 2 sin^2(pi y / 2) for any finite binary32 y (exact reduction; |y| >= 2^24
+// gives 1); stands in for CUDA's cospif
+__device__ __forceinline__ float cospi_f(float y) {
+    const float h = 0.5f * y;
+    const float s = sinpi_sq(h);
+    const float t = 2.0f * s;
+    return 1.0f - t;
+}
+
+// e^x = expm1(x) + 1; stands in for CUDA's expf
+__device__ __forceinline__ float exp_f(float x) { return expm1_f(x) + 1.0f; }
+
+// sum_{k=0}^{20} 0.5^k cospi(tv 3^k), tv = 2 (z + 0.5) and each product rounded to binary32
+// as the reference's cospif argument is (SG-EMTF_DE.cu:1012-1022); 3^k by the binary32
+// recurrence p <- 3 p, whose values are the correctly rounded powers
+__device__ __forceinline__ float weier_ref_term(float z) {
+    constexpr float P3R[21] = {1.f, 3.f, 9.f, 27.f, 81.f, 243.f, 729.f, 2187.f, 6561.f, 19683.f,
+                               59049.f, 177147.f, 531441.f, 1594323.f, 4782969.f, 14348907.f,
+                               43046721.f, 129140163.f, 387420489.f, 1162261467.f, 3486784401.f};
+    const float zh = z + 0.5f;
+    const float tv = 2.0f * zh;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k <= 20; ++k) {
+        const float y = tv * P3R[k];
+        const float c = cospi_f(y);
+        const float t = pow2i(-k) * c;
+        acc = acc + t;
+    }
+    return acc;
+}
+
 // 16-bit uniform u = (v + 1) 2^-16 in (0,1]: half 0 = high 16 bits of w, half 1 = low.
 // u > CR  <=>  v >= floor(CR 2^16) for every binary32 CR in [0,1] (both sides exact), so
 // the crossover test compares the raw 16-bit value with a precomputed threshold.

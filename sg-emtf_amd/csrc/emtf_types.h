@@ -4,9 +4,15 @@
 
 namespace emtf {
 
+// 0..7: this engine's binary32 forms; 8..13: the reference's own formulas and constants
+// (evaluation(), SG-EMTF_DE.cu:834-1069).  12 runs the Schwefel body with the reference's
+// constant in the table; 13 the reference Griewank body with +inf divisors (the GA copy's
+// disabled product, SG-EMTF.cu:1029: every factor cos(z / inf) = 1).
 enum Func : int32_t {
     F_SPHERE = 0, F_ROSENBROCK = 1, F_ACKLEY = 2, F_RASTRIGIN = 3,
-    F_GRIEWANK = 4, F_WEIERSTRASS = 5, F_SCHWEFEL = 6, F_ELLIPTIC = 7
+    F_GRIEWANK = 4, F_WEIERSTRASS = 5, F_SCHWEFEL = 6, F_ELLIPTIC = 7,
+    F_REF_ACKLEY = 8, F_REF_RASTRIGIN = 9, F_REF_GRIEWANK = 10, F_REF_WEIERSTRASS = 11,
+    F_REF_SCHWEFEL = 12, F_REF_GRIEWANK_GA = 13
 };
 enum Algo : int32_t { A_DE_RAND_1_BIN = 0, A_DE_CURRENT_1_BIN = 1, A_GA_SBX_PM = 2 };
 

@@ -90,7 +90,7 @@ __device__ __forceinline__ float clamp_box(float v, float lo, float hi) {
 // ---------------------------------------------------------------------------
 template <int NT, int FUNC, bool SKIP = false>
 __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const float* tab, int h) {
-    float s1 = 0.0f, s2 = 0.0f;
+    float s1 = 0.0f, s2 = (FUNC == F_REF_GRIEWANK) ? 1.0f : 0.0f;
     if constexpr (FUNC == F_ROSENBROCK) {
         float nb[NT][4];
 #pragma unroll
@@ -163,6 +163,31 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
                     const float tv = sq * 0.159154937f;
                     const float v = z * sin2pi(tv);
                     if (j < D) s1 = s1 + v;
+                } else if constexpr (FUNC == F_REF_ACKLEY) {
+                    // the reference's terms: z^2 and cospif(2 z)
+                    const float v = z * z;
+                    const float tz = 2.0f * z;
+                    const float c = cospi_f(tz);
+                    if (j < D) { s1 = s1 + v; s2 = s2 + c; }
+                } else if constexpr (FUNC == F_REF_RASTRIGIN) {
+                    // z^2 - 10 cospif(2 z) + 10
+                    const float zz = z * z;
+                    const float tz = 2.0f * z;
+                    const float c = cospi_f(tz);
+                    const float t = 10.0f * c;
+                    float v = zz - t;
+                    v = v + 10.0f;
+                    if (j < D) s1 = s1 + v;
+                } else if constexpr (FUNC == F_REF_GRIEWANK) {
+                    // z^2 and the factor cosf(z / sqrtf(j + 1)), cos q = 1 - 2 sin^2(q / 2)
+                    const float v = z * z;
+                    const float q = z / tab[j];
+                    const float tq = q * 0.159154937f;
+                    const float t = 2.0f * sinpi_sq(tq);
+                    const float c = 1.0f - t;
+                    if (j < D) { s1 = s1 + v; s2 = s2 * c; }
+                } else if constexpr (FUNC == F_REF_WEIERSTRASS) {
+                    if (j < D) s1 = s1 + weier_ref_term(z);
                 }
             }
     }
@@ -190,8 +215,34 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
         const float s4 = S / 4000.0f;
         return s4 + Qv;
     } else if constexpr (FUNC == F_SCHWEFEL) {
-        const float cd = 418.982887f * Df;
+        // the constant from the task table: 418.982887 (textbook) or 418.9829 (reference)
+        const float cd = tab[0] * Df;
         return cd - S;
+    } else if constexpr (FUNC == F_REF_ACKLEY) {
+        // -20 expf(-0.2 sqrtf(S / D)) - expf(C / D) + 20 + E with the reference's double
+        // literals -0.2 and E = 2.718282 (SG-EMTF_DE.cu:913)
+        const float o2 = __shfl_xor(s2, 32);
+        const float C = s2 + o2;
+        const float q = S / Df;
+        const float r = sqrtf(q);
+        const float a = (float)(-0.2 * (double)r);
+        const float t1 = -20.0f * exp_f(a);
+        const float c2 = C / Df;
+        const float e2 = exp_f(c2);
+        float u = t1 - e2;
+        u = u + 20.0f;
+        return (float)((double)u + 2.718282);
+    } else if constexpr (FUNC == F_REF_GRIEWANK) {
+        // 1 + S / 4000 - prod (the halves' partial products multiplied once)
+        const float o2 = __shfl_xor(s2, 32);
+        const float P = s2 * o2;
+        const float s4 = S / 4000.0f;
+        const float u = 1.0f + s4;
+        return u - P;
+    } else if constexpr (FUNC == F_REF_WEIERSTRASS) {
+        // sum - D W with the reference's constant W in the task table
+        const float dw = Df * tab[0];
+        return S - dw;
     } else {
         return S;
     }
@@ -508,7 +559,8 @@ __device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs
     }
     // the padded row groups are skipped for the two costliest epilogues only (measured:
     // the branch costs more than it saves for the others)
-    return objective<NT, FUNC, FUNC == F_WEIERSTRASS || FUNC == F_SCHWEFEL>(acc, g.D, L.tab, h);
+    return objective<NT, FUNC, FUNC == F_WEIERSTRASS || FUNC == F_SCHWEFEL || FUNC == F_REF_WEIERSTRASS>(acc, g.D,
+                                                                                                     L.tab, h);
 }
 
 template <int KB>
@@ -1172,6 +1224,11 @@ __global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(E
     case F_WEIERSTRASS: epoch_body<KB, F_WEIERSTRASS, ALGO>(a, smem, task, td); break;
     case F_SCHWEFEL: epoch_body<KB, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
     case F_ELLIPTIC: epoch_body<KB, F_ELLIPTIC, ALGO>(a, smem, task, td); break;
+    case F_REF_ACKLEY: epoch_body<KB, F_REF_ACKLEY, ALGO>(a, smem, task, td); break;
+    case F_REF_RASTRIGIN: epoch_body<KB, F_REF_RASTRIGIN, ALGO>(a, smem, task, td); break;
+    case F_REF_GRIEWANK: case F_REF_GRIEWANK_GA: epoch_body<KB, F_REF_GRIEWANK, ALGO>(a, smem, task, td); break;
+    case F_REF_WEIERSTRASS: epoch_body<KB, F_REF_WEIERSTRASS, ALGO>(a, smem, task, td); break;
+    case F_REF_SCHWEFEL: epoch_body<KB, F_SCHWEFEL, ALGO>(a, smem, task, td); break;
     default: break;
     }
 }
@@ -1650,6 +1707,11 @@ __global__ __launch_bounds__(WT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     case F_WEIERSTRASS: wide_body<NT, F_WEIERSTRASS, ALGO, WT, QL>(a, smem, task, td); break;
     case F_SCHWEFEL: wide_body<NT, F_SCHWEFEL, ALGO, WT, QL>(a, smem, task, td); break;
     case F_ELLIPTIC: wide_body<NT, F_ELLIPTIC, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_REF_ACKLEY: wide_body<NT, F_REF_ACKLEY, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_REF_RASTRIGIN: wide_body<NT, F_REF_RASTRIGIN, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_REF_GRIEWANK: case F_REF_GRIEWANK_GA: wide_body<NT, F_REF_GRIEWANK, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_REF_WEIERSTRASS: wide_body<NT, F_REF_WEIERSTRASS, ALGO, WT, QL>(a, smem, task, td); break;
+    case F_REF_SCHWEFEL: wide_body<NT, F_SCHWEFEL, ALGO, WT, QL>(a, smem, task, td); break;
     default: break;
     }
 }

@@ -28,7 +28,8 @@ class Params(ctypes.Structure):
                 ("migrate_interval", ctypes.c_int32), ("migrate_count", ctypes.c_int32),
                 ("transfer_interval", ctypes.c_int32), ("transfer_count", ctypes.c_int32),
                 ("transfer_cr", ctypes.c_float), ("seed", ctypes.c_uint64), ("polish_interval", ctypes.c_int32),
-                ("polish_steps", ctypes.c_int32), ("polish_topk", ctypes.c_int32), ("schedule", ctypes.c_int32), ("tune", ctypes.c_int32)]
+                ("polish_steps", ctypes.c_int32), ("polish_topk", ctypes.c_int32), ("schedule", ctypes.c_int32), ("tune", ctypes.c_int32),
+                ("formulas", ctypes.c_int32)]
 
 
 class EmtfError(RuntimeError):

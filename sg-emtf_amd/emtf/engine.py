@@ -17,11 +17,14 @@ FUNC_IDS = {"sphere": 0, "rosenbrock": 1, "ackley": 2, "rastrigin": 3, "griewank
 SHIFT, ROTATE, DIAG = 0, 1, 2
 DE_RAND_1_BIN, DE_CURRENT_1_BIN, GA_SBX_PM = 0, 1, 2
 SCHEDULE_AUTO, SCHEDULE_EPOCHS, SCHEDULE_PERSISTENT = 0, 1, 2
+# objective formulas: this engine's binary32 forms, or the reference's own (its DE file's
+# evaluation(), or the GA file's copy whose Griewank has no product)
+FORMULAS_TEXTBOOK, FORMULAS_REFERENCE_DE, FORMULAS_REFERENCE_GA = 0, 1, 2
 
 DEFAULT_PARAMS = dict(algo=DE_RAND_1_BIN, F=0.5, CR=0.9, eta_c=2.0, eta_m=5.0, pm=1.0, n_islands=1,
                       exchange_interval=0, migrate_interval=0, migrate_count=0, transfer_interval=0,
                       transfer_count=0, transfer_cr=0.0, seed=0, polish_interval=0, polish_steps=0, polish_topk=0,
-                      schedule=0, tune=0)
+                      schedule=0, tune=0, formulas=0)
 
 
 def _fp(a):
