@@ -127,6 +127,16 @@ int emtf_get_best(emtf_handle* h, float* fitness, float* x_best, int32_t x_strid
  * messageTransfer with BESTNUM_PERTASK = k (SG-EMTF.cu:60, :462-486, :1601-1615). */
 int emtf_get_elites(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t x_stride);
 
+/* emtf_get_elites without any device allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The ranked rows are staged in a fixed
+ * array of 2^20 floats (4 MiB) that the library keeps in every GPU's memory until the
+ * process ends: n_tasks * k * (x_stride + 1) floats at most (x_stride counts 0 when x is
+ * NULL); larger requests return EMTF_E_INVALID.  Every call on one GPU shares that array
+ * and overwrites it, so calls that overlap on one GPU (from several threads or handles)
+ * race.  Same results as emtf_get_elites; synchronises. */
+int emtf_get_elites_noalloc(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t x_stride);
+
 /* Convergence trace recorded on the device (the reference's per-task record, SG-EMTF_DE.cu:
  * 1793-1811: best at generation g0 and after every `interval` generations, then the worst
  * of the population at g0).  Records point 0 now (g0 = the handle's generation) and then,

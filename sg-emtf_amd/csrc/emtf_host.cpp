@@ -64,6 +64,7 @@ extern "C" int emtf_launch_trace(const emtf::TaskDev* tasks, int n_tasks, const 
 extern "C" int emtf_launch_draws(uint64_t seed, int n, const uint32_t* addr, uint32_t* out, hipStream_t stream);
 extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const float* X, const float* fit, int k,
                                   int stride, float* out_x, float* out_f, hipStream_t stream);
+extern "C" int emtf_scratch_resolve(int device, float** out, size_t* capacity);
 
 static const int kKB[] = {2, 4, 7, 8, 12, 13, 16};
 
@@ -228,6 +229,7 @@ struct emtf_handle {
     // elite export staging, kept between calls (grown on demand)
     float* d_elite = nullptr;
     size_t elite_cap = 0;        // floats
+    int device = 0;              // the GPU the handle lives on
 };
 
 static void free_handle(emtf_handle* h) {
@@ -801,6 +803,19 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         return fail(EMTF_E_UNSUPPORTED, "persistent schedule is not available in this build");
     }
 
+    {
+        // the device this handle lives on, and (once per device) the address of its fixed
+        // staging array for emtf_get_elites_noalloc
+        float* sp = nullptr;
+        size_t cap = 0;
+        hipError_t e0 = hipGetDevice(&h->device);
+        if (e0 == hipSuccess) e0 = (hipError_t)emtf_scratch_resolve(h->device, &sp, &cap);
+        if (e0 != hipSuccess) {
+            free_handle(h);
+            return fail(EMTF_E_DEVICE, std::string("device / staging array: ") + hipGetErrorString(e0));
+        }
+    }
+
     std::vector<float> Q((size_t)nQ, 0.0f), V((size_t)nV, 0.0f), TB((size_t)nV, 0.0f);
     for (int t = 0; t < n_tasks; ++t) {
         const emtf_task_desc& d = tasks[t];
@@ -1067,9 +1082,37 @@ int emtf_get_elites(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t
     }
     float* df = h->d_elite;
     float* dx = x ? h->d_elite + nf : nullptr;
-    if (e == hipSuccess && x) e = hipMemsetAsync(dx, 0, nx * sizeof(float), h->stream);
     if (e == hipSuccess)
         e = (hipError_t)emtf_launch_elites(h->d_tasks, h->T, h->d_X, h->d_fit, k, x_stride, dx, df, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && fitness) e = hipMemcpy(fitness, df, nf * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && x) e = hipMemcpy(x, dx, nx * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(EMTF_E_DEVICE, std::string("elite export: ") + hipGetErrorString(e));
+    return EMTF_OK;
+}
+
+int emtf_get_elites_noalloc(emtf_handle* h, int32_t k, float* fitness, float* x, int32_t x_stride) {
+    g_err.clear();
+    if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
+    if (k < 1) return fail(EMTF_E_INVALID, "k < 1");
+    if (x && x_stride < h->max_dim) return fail(EMTF_E_INVALID, "x_stride < max dim");
+    for (const emtf::TaskDev& td : h->tasks)
+        if (k > td.pop) return fail(EMTF_E_INVALID, "k exceeds a task's population");
+    float* base = nullptr;
+    size_t cap = 0;
+    // resolved at create for this device: a table lookup here, no allocation
+    hipError_t e = (hipError_t)emtf_scratch_resolve(h->device, &base, &cap);
+    if (e != hipSuccess) return fail(EMTF_E_DEVICE, std::string("staging array: ") + hipGetErrorString(e));
+    const size_t nx = x ? (size_t)h->T * k * x_stride : 0, nf = (size_t)h->T * k;
+    if (nf + nx > cap) {
+        char buf[200];
+        snprintf(buf, sizeof buf, "elite export needs %zu floats of staging; the fixed array holds %zu "
+                 "(n_tasks * k * (x_stride + 1) at most)", nf + nx, cap);
+        return fail(EMTF_E_INVALID, buf);
+    }
+    float* df = base;
+    float* dx = x ? base + nf : nullptr;
+    e = (hipError_t)emtf_launch_elites(h->d_tasks, h->T, h->d_X, h->d_fit, k, x_stride, dx, df, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && fitness) e = hipMemcpy(fitness, df, nf * sizeof(float), hipMemcpyDeviceToHost);
     if (e == hipSuccess && x) e = hipMemcpy(x, dx, nx * sizeof(float), hipMemcpyDeviceToHost);

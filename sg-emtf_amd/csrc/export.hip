@@ -9,10 +9,20 @@
 // engine uses everywhere -- and writes the first k rows to a dense [task][k] block.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <mutex>
+
 #include "emtf_types.h"
 #include "emtf_math.h"
 
 namespace emtf {
+
+// Staging of emtf_get_elites_noalloc: a fixed array in every GPU's memory for the life of
+// the process.  Host code never uses its name as an address: emtf_scratch_resolve() asks
+// the runtime for each device's copy once, with that device current, and callers pass the
+// returned pointer to the kernels.
+constexpr size_t kScratchFloats = size_t(1) << 20;  // 4 MiB
+static __device__ __align__(256) float scratch[kScratchFloats];
 
 __global__ __launch_bounds__(256) void elite_kernel(const TaskDev* __restrict__ tasks, const float* __restrict__ X,
                                                     const float* __restrict__ fit, int k, int stride,
@@ -36,9 +46,10 @@ __global__ __launch_bounds__(256) void elite_kernel(const TaskDev* __restrict__ 
     __syncthreads();
     const int kk = k < P ? k : P;
     if (out_x) {
-        for (int e = tid; e < kk * D; e += blockDim.x) {
-            const int r = e / D, d = e - r * D;
-            out_x[((int64_t)t * k + r) * stride + d] = X[td.x_off + (int64_t)ord[r] * D + d];
+        // whole rows of the stride: genes, then zeros past the task's dimension
+        for (int e = tid; e < kk * stride; e += blockDim.x) {
+            const int r = e / stride, d = e - r * stride;
+            out_x[((int64_t)t * k + r) * stride + d] = d < D ? X[td.x_off + (int64_t)ord[r] * D + d] : 0.0f;
         }
     }
     for (int r = tid; r < kk; r += blockDim.x) out_f[(int64_t)t * k + r] = fit[td.fit_off + ord[r]];
@@ -103,6 +114,32 @@ extern "C" int emtf_launch_draws(uint64_t seed, int n, const uint32_t* addr, uin
     hipLaunchKernelGGL(emtf::draws_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, (uint32_t)seed,
                        (uint32_t)(seed >> 32), n, addr, out);
     return (int)hipGetLastError();
+}
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+float* g_scratch[kMaxDevices];  // each device's copy of emtf::scratch, once resolved
+}  // namespace
+
+// Device address of `device`'s staging array (resolved once per device, with that device
+// current) and its capacity in floats.  Returns a hipError_t value.
+extern "C" int emtf_scratch_resolve(int device, float** out, size_t* capacity) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return (int)hipErrorInvalidDevice;
+    if (!g_scratch[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        void* p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(emtf::scratch));
+        if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return (int)e;
+        g_scratch[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch[device];
+    *capacity = emtf::kScratchFloats;
+    return (int)hipSuccess;
 }
 
 extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const float* X, const float* fit, int k,
