@@ -86,10 +86,15 @@ typedef struct emtf_params {
  * Rosenbrock and Elliptic are the same in every mode.  Gradient polish needs TEXTBOOK. */
 enum { EMTF_FORMULAS_TEXTBOOK = 0, EMTF_FORMULAS_REFERENCE_DE = 1, EMTF_FORMULAS_REFERENCE_GA = 2 };
 
-/* How emtf_run launches work.  EPOCHS (what AUTO selects): one launch per transfer
- * interval, the kernel boundary ordering the elite mailbox.  PERSISTENT (one launch
- * per call, the mailbox ordered by device-side completion stamps) is reserved and
- * returns EMTF_E_UNSUPPORTED at create in this build. */
+/* How emtf_run launches work.  Work is cut into epochs that end at every transfer
+ * boundary, the kernel boundary ordering the elite mailbox.  EPOCHS: each epoch's kernels
+ * (one per launch unit, forked over up to four streams and joined) are launched directly.
+ * AUTO: the same epochs, replayed from a hipGraph that the handle captures once (and again
+ * when the epoch length changes): epochs of one whole transfer interval (or of any length
+ * without transfer) whose end carries no trace point or polish; the kernels read their
+ * first generation from a device counter that the graph advances.  Same results either way.
+ * PERSISTENT (one launch per call, the mailbox ordered by device-side completion stamps) is
+ * reserved and returns EMTF_E_UNSUPPORTED at create in this build. */
 enum { EMTF_SCHEDULE_AUTO = 0, EMTF_SCHEDULE_EPOCHS = 1, EMTF_SCHEDULE_PERSISTENT = 2 };
 
 typedef struct emtf_handle emtf_handle;

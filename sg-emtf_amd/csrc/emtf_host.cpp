@@ -65,6 +65,7 @@ extern "C" int emtf_launch_draws(uint64_t seed, int n, const uint32_t* addr, uin
 extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const float* X, const float* fit, int k,
                                   int stride, float* out_x, float* out_f, hipStream_t stream);
 extern "C" int emtf_scratch_resolve(int device, float** out, size_t* capacity);
+extern "C" int emtf_launch_gen_counter(int32_t* gen, int32_t value, int32_t add, hipStream_t stream);
 
 static const int kKB[] = {2, 4, 7, 8, 12, 13, 16};
 
@@ -230,6 +231,15 @@ struct emtf_handle {
     float* d_elite = nullptr;
     size_t elite_cap = 0;        // floats
     int device = 0;              // the GPU the handle lives on
+    // replayed epochs: one epoch's launches (fork, every launch unit, join, counter advance)
+    // captured once into a hipGraph and replayed while the epochs keep that length
+    bool use_graph = false;
+    int32_t* d_gen = nullptr;    // device generation counter the replayed kernels start from
+    bool gen_valid = false;      // d_gen == gen (false after directly launched epochs)
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_len = 0;           // generations per replayed epoch
 };
 
 static void free_handle(emtf_handle* h) {
@@ -247,6 +257,10 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_trace);
     (void)hipFree(h->d_trace_worst);
     (void)hipFree(h->d_elite);
+    (void)hipFree(h->d_gen);
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
 
     for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
@@ -329,8 +343,9 @@ static bool publish_at(const emtf_handle* h, int g) {
 // Epoch g0 -> g1 (or the initial evaluation) of every bucket, each on its stream;
 // the polish of generation g1 follows in its own kernel, which then also
 // publishes the elites.
-static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) {
+static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream, const int32_t* gen_dev = nullptr) {
     emtf::EpochArgs a{};
+    a.gen_dev = gen_dev;
     a.tasks = h->d_tasks;
     a.n_tasks = h->T;
     a.g0 = g0;
@@ -376,6 +391,48 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream) 
             HIP_OR_FAIL(hipStreamWaitEvent(stream, h->ev_join[k], 0));
         }
     }
+    return EMTF_OK;
+}
+
+// Capture one epoch of `len` generations (its first at g0) into the handle's graph: the
+// fork, every launch unit and the join as launch() issues them, with the kernels starting
+// from the device counter, then the counter's advance by len.  Replaces a previous graph.
+static int capture_epoch(emtf_handle* h, int g0, int len) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_len = 0;
+    if (!h->cap_stream) HIP_OR_FAIL(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+    HIP_OR_FAIL(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeRelaxed));
+    int rc = launch(h, g0, g0 + len, 0, h->cap_stream, h->d_gen);
+    if (rc == EMTF_OK) {
+        const int e = emtf_launch_gen_counter(h->d_gen, len, 1, h->cap_stream);
+        if (e != 0) rc = fail(EMTF_E_DEVICE, std::string("counter: ") + hipGetErrorString((hipError_t)e));
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
+    if (rc != EMTF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(EMTF_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    h->graph = g;
+    HIP_OR_FAIL(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+    h->graph_len = len;
+    return EMTF_OK;
+}
+
+// Epoch g0 -> g0 + len replayed from the graph (captured on first use or when len changes).
+static int replay_epoch(emtf_handle* h, int g0, int len, hipStream_t s) {
+    if (!h->graph_exec || h->graph_len != len)
+        if (const int rc = capture_epoch(h, g0, len); rc != EMTF_OK) return rc;
+    if (!h->gen_valid) {
+        const int e = emtf_launch_gen_counter(h->d_gen, g0, 0, s);
+        if (e != 0) return fail(EMTF_E_DEVICE, std::string("counter: ") + hipGetErrorString((hipError_t)e));
+    }
+    HIP_OR_FAIL(hipGraphLaunch(h->graph_exec, s));
+    h->gen_valid = true;
     return EMTF_OK;
 }
 
@@ -857,6 +914,11 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(tasks)"); }
     e = hipMalloc((void**)&h->d_order, sizeof(int) * n_tasks);
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(order)"); }
+    e = hipMalloc((void**)&h->d_gen, sizeof(int32_t));
+    if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(generation counter)"); }
+    // replayed epochs unless the caller asks for direct launches; the polish kernel's epochs
+    // are launched directly
+    h->use_graph = prm->schedule == EMTF_SCHEDULE_AUTO && !want_polish;
 #define CP(dst, src, bytes)                                                                    \
     do {                                                                                       \
         hipError_t e2 = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);                     \
@@ -907,10 +969,21 @@ int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
             const int nb = h->trace_g0 + ((h->gen - h->trace_g0) / TR + 1) * TR;
             if (nb < g1) g1 = nb;
         }
-        const int rc = launch(h, h->gen, g1, 0, s);
+        const bool trace_here = TR > 0 && (g1 - h->trace_g0) % TR == 0;
+        // a replayable epoch: no trace point or polish at its end and, with transfer, one
+        // whole transfer interval (so every replay ends on a boundary and publishes)
+        const bool replay = h->use_graph && !trace_here && !polish_at(h, g1) &&
+                            (!xfer || (h->gen % TI == 0 && g1 - h->gen == TI));
+        int rc;
+        if (replay) {
+            rc = replay_epoch(h, h->gen, g1 - h->gen, s);
+        } else {
+            rc = launch(h, h->gen, g1, 0, s);
+            h->gen_valid = false;
+        }
         if (rc != EMTF_OK) return rc;
         h->gen = g1;
-        if (TR > 0 && (g1 - h->trace_g0) % TR == 0) {
+        if (trace_here) {
             const int e = emtf_launch_trace(h->d_tasks, h->T, h->d_fit, h->d_trace + (int64_t)h->trace_points * h->T,
                                             nullptr, s);
             if (e != 0) return fail(EMTF_E_DEVICE, std::string("trace: ") + hipGetErrorString((hipError_t)e));

@@ -57,6 +57,9 @@ struct EpochArgs {
     const float* Qw;   // padded Q images of the wide tasks
     EngineParams p;
     int32_t publish;       // write the task's elites to the mailbox after generation g1
+    // Replayed epochs (hipGraph): when set, the epoch starts at *gen_dev instead of g0 and
+    // keeps its length g1 - g0 (kernel arguments are frozen at capture; the counter is not)
+    const int32_t* gen_dev;
     int32_t polish_waves;  // polish kernel: waves working on elites (1..4)
     int32_t polish_cpe;    // polish kernel: MFMA columns per elite (power of two, 1..32)
 };

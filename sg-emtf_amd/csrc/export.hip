@@ -102,7 +102,18 @@ __global__ __launch_bounds__(256) void trace_kernel(const TaskDev* __restrict__ 
     }
 }
 
+// The device generation counter of replayed epochs: set (after directly launched
+// epochs) or advanced by one epoch (the last node of a captured epoch).
+__global__ void gen_counter_kernel(int32_t* gen, int32_t value, int32_t add) {
+    if (threadIdx.x == 0) gen[0] = add ? gen[0] + value : value;
+}
+
 }  // namespace emtf
+
+extern "C" int emtf_launch_gen_counter(int32_t* gen, int32_t value, int32_t add, hipStream_t stream) {
+    hipLaunchKernelGGL(emtf::gen_counter_kernel, dim3(1), dim3(64), 0, stream, gen, value, add);
+    return (int)hipGetLastError();
+}
 
 extern "C" int emtf_launch_trace(const emtf::TaskDev* tasks, int n_tasks, const float* fit, float* out_best,
                                  float* out_worst, hipStream_t stream) {

@@ -82,6 +82,16 @@ __device__ __forceinline__ float clamp_box(float v, float lo, float hi) {
     return __builtin_fminf(__builtin_fmaxf(v, lo), hi);
 }
 
+// A replayed epoch (hipGraph) starts at the device generation counter: its launch
+// arguments were frozen when the graph was captured, the counter advances every replay.
+__device__ __forceinline__ void rebase_epoch(EpochArgs& a) {
+    if (a.gen_dev) {
+        const int g0 = *a.gen_dev;
+        a.g1 = g0 + (a.g1 - a.g0);
+        a.g0 = g0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Objective of the lane's individual from its MFMA accumulators.
 // Lane (col, h) holds z_j for j = 32 jt + 8 (r >> 2) + 4 h + (r & 3); the two
@@ -1213,6 +1223,7 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
 template <int KB, int ALGO>
 __global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(EpochArgs a, const int* __restrict__ order) {
     extern __shared__ float smem[];
+    rebase_epoch(a);
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
     switch (td.func) {
@@ -1696,6 +1707,7 @@ template <int NT, int ALGO, int WT, bool QL, int WPE>
 __global__ __launch_bounds__(WT) __attribute__((amdgpu_waves_per_eu(WPE))) void wide_epoch_kernel(
     EpochArgs a, const int* __restrict__ order) {
     extern __shared__ float smem[];
+    rebase_epoch(a);
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
     switch (td.func) {

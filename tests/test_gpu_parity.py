@@ -186,6 +186,35 @@ def test_autotune_is_bit_exact(oracle_mod, tune):
         _compare(gpu, orc, f"tune={tune} gen {orc.generation}")
 
 
+@pytest.mark.parametrize("schedule", [emtf.SCHEDULE_AUTO, emtf.SCHEDULE_EPOCHS])
+def test_graph_replay_bit_exact(oracle_mod, schedule):
+    """Epochs replayed from the captured hipGraph (AUTO) and launched directly (EPOCHS) both
+    equal the oracle: a batch of several launch units forked over streams, runs that start on
+    and off transfer boundaries (replayed and direct epochs interleaved), a trace started in
+    between, and host reads between replays."""
+    dims = [20, 50, 50, 150, 100, 33, 64, 20] * 3
+    pops = [32, 128, 256, 64, 96, 64, 160, 32] * 3
+    tasks = tg.benchmark_tasks(len(dims), 0, 0, rotate=True, seed=21, dims=dims, pops=pops)
+    gpu, orc = _pair(tasks, oracle_mod, seed=17, F=0.5, CR=0.9, transfer_interval=10, transfer_count=3,
+                     schedule=schedule)
+    for k, step in enumerate((10, 10, 5, 5, 20, 13, 7, 10)):
+        if k == 5:
+            gpu.trace_start(10, 4)
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"schedule {schedule} gen {orc.generation}")
+
+
+def test_graph_replay_without_transfer(oracle_mod):
+    """No transfer: every epoch is replayable; a new run length recaptures the graph."""
+    tasks = tg.benchmark_tasks(12, 40, 64, rotate=True, seed=5)
+    gpu, orc = _pair(tasks, oracle_mod, seed=3, F=0.5, CR=0.9)
+    for step in (7, 7, 3, 7, 7):
+        gpu.run(step)
+        orc.run(step)
+        _compare(gpu, orc, f"no transfer gen {orc.generation}")
+
+
 def test_wide_512_threads_ga(oracle_mod):
     """The 512-thread wide build (tune=2 selects it for populations above four tiles)."""
     tasks = tg.benchmark_tasks(6, 170, 224, rotate=True, seed=13)
