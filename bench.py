@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--only-func", default=None, help="give every task this objective (per-objective timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters)")
+    ap.add_argument("--no-study", action="store_true",
+                    help="plain workload: skip the one-seed time-to-1e-6 study (adjoint polish vs plain DE)")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "epochs"],
+                    help="auto: epochs replayed from a captured hipGraph; epochs: launched directly")
     return ap.parse_args()
 
 
@@ -156,6 +160,54 @@ def _time_to(t_ms, mean_err, target):
     import numpy as np
     hit = np.nonzero(mean_err <= target)[0]
     return float(t_ms[hit[0]]) if len(hit) else None
+
+
+NAMES = ["sphere", "rosenbrock", "ackley", "rastrigin", "griewank", "weierstrass", "schwefel", "elliptic"]
+
+
+def time_to_target_study(seed=0, gens=3000, chunk=50, target=1e-6):
+    """The second metric on one seed: the adjoint-polish configuration (1024 rotated tasks,
+    D = 100, pop 128, DE/rand/1/bin, 4 elites every 20 gens; polish: 5 Armijo steps on the
+    top 4 elites every 50 gens) run plain and polished.  Error = float64 f(best) - f* of
+    every task (Schwefel excluded: its minimum lies outside the box).  Per objective: GPU
+    time (kernels, HIP events) until the mean error <= target, for both variants, and the
+    time ratio polished / plain where both reach it."""
+    import numpy as np
+    import emtf
+    from emtf import tasks as tg
+    tasks = tg.benchmark_tasks(1024, 100, 128, rotate=True, seed=seed)
+    funcs = np.array([t["func"] for t in tasks])
+    base = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=20, transfer_count=4, seed=seed)
+    runs = {}
+    for mode in ("plain", "polish"):
+        p = dict(base)
+        if mode == "polish":
+            p.update(polish_interval=50, polish_steps=5, polish_topk=4)
+        runs[mode] = _converge(tasks, p, gens, chunk, target)
+    per = {}
+    ratios = []
+    for f, n in enumerate(NAMES):
+        if n == "schwefel":
+            continue
+        row = {}
+        for mode, (t_ms, errs) in runs.items():
+            m = np.mean(errs[:, funcs == f], axis=1)
+            row[mode + "_ms"] = _time_to(t_ms, m, target)
+            row[mode + "_final_mean_error"] = float(m[-1])
+        if row["plain_ms"] and row["polish_ms"]:
+            row["ratio"] = row["polish_ms"] / row["plain_ms"]
+            ratios.append(row["ratio"])
+        per[n] = row
+    live = funcs != NAMES.index("schwefel")
+    full = {m: float(r[0][-1]) for m, r in runs.items()}
+    mean_all = {m: _time_to(r[0], np.nanmean(r[1][:, live], axis=1), target) for m, r in runs.items()}
+    return {"config": "1024 rotated tasks, D=100, pop 128, 3000 gens, seed %d; polish 5 Armijo steps x top 4 "
+                      "every 50 gens" % seed,
+            "target_mean_error": target, "per_objective": per,
+            "ratio_geomean_over_objectives_reached_by_both": float(np.exp(np.mean(np.log(ratios)))) if ratios else None,
+            "objectives_reached_by_both": [n for n, r in per.items() if "ratio" in r],
+            "mean_error_all_objectives_ms": mean_all,
+            "gpu_ms_full_run": full, "polish_overhead_frac": full["polish"] / full["plain"] - 1.0}
 
 
 def bench_polish(a):
@@ -301,12 +353,13 @@ def main():
     else:
         funcs = [emtf.FUNC_IDS[a.only_func]] * a.tasks if a.only_func else None
         tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed, funcs=funcs)
+    sched = emtf.SCHEDULE_AUTO if a.schedule == "auto" else emtf.SCHEDULE_EPOCHS
     if a.algo == "ga":
         params = dict(algo=emtf.GA_SBX_PM, eta_c=2.0, eta_m=5.0, pm=1.0, transfer_interval=a.interval,
-                      transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0)
+                      transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0, schedule=sched)
     else:
         params = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval,
-                      transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0)
+                      transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0, schedule=sched)
     algo_name = "GA SBX(eta_c 2)+PM(eta_m 5, pm 1)" if a.algo == "ga" else "DE/rand/1/bin F=0.5 CR=0.9"
     eng = emtf.Engine(tasks, **params)
     for _ in range(a.warmup):
@@ -342,6 +395,9 @@ def main():
     cpu = None
     if a.cpu_gens > 0:
         cpu = cpu_baseline(tasks, params, a.cpu_gens, min(a.cpu_tasks, a.tasks))
+    study = None
+    if not uneven and not a.no_study and not a.only_func:
+        study = time_to_target_study(seed=a.seed)
 
     out = {
         "metric": METRIC, "value": value, "unit": "task-generations/s", "n_gpus": 1,
@@ -361,6 +417,8 @@ def main():
                                     + (", summed over tasks" if uneven else ""),
                      "avg_launch_ms": gpu_ms / a.steps},
         "cpu_baseline": cpu,
+        "time_to_1e-6": study,
+        "launch": ("epochs replayed from a captured hipGraph" if a.schedule == "auto" else "epochs launched directly"),
         "wall_s": wall,
         "mean_best_fitness": float(best.mean()),
     }

@@ -775,22 +775,21 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
             int max_pop = 0;
             for (int u = 0; u < n_tasks; ++u)
                 if (bucket_of[u] == bucket_of[t]) max_pop = std::max(max_pop, tasks[u].pop);
-            // spread the elites over the four waves and let the spare MFMA columns try
-            // further step sizes: cpe columns per elite, the largest power of two <= 32
-            // with k elites fitting 4 waves of 32 / cpe
+            // the elites go through in groups of 32 / cpe slots (all four waves on each
+            // group): cpe columns per elite, the largest power of two <= 32 with every
+            // elite in one group when they fit (up to 32), halved slots while LDS is short
             const int k = std::min(prm->polish_topk, max_pop);
-            int cpe = 32;
-            while (cpe > 1 && cpe * k > 128) cpe /= 2;
-            const int epw = 32 / cpe;
-            int w = std::min(4, (k + epw - 1) / epw);
-            while (w > 1 && 4 * plds(max_pop, w, cpe) > kMaxLdsBytes) --w;
-            if (4 * plds(max_pop, w, cpe) > kMaxLdsBytes) {
+            int slots = 1;  // a power of two >= min(k, 32)
+            while (slots < std::min(k, 32)) slots *= 2;
+            int cpe = 32 / slots;
+            while (cpe < 32 && 4 * plds(max_pop, 4, cpe) > kMaxLdsBytes) cpe *= 2;
+            if (4 * plds(max_pop, 4, cpe) > kMaxLdsBytes) {
                 free_handle(h);
                 return fail(EMTF_E_UNSUPPORTED, "polish staging exceeds LDS");
             }
-            b.polish_waves = w;
+            b.polish_waves = 4;
             b.polish_cpe = cpe;
-            b.polish_lds_bytes = 4 * plds(max_pop, w, cpe);
+            b.polish_lds_bytes = 4 * plds(max_pop, 4, cpe);
         }
     }
     // Launch units: each bucket's tasks, most expensive first.  With several buckets, one

@@ -804,167 +804,136 @@ __host__ __device__ inline int obj_lds_floats() {
     return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * PB * ZS;
 }
 
+// Per-point factors of d f / d z: Ackley's two (from the sums over the point's z).
+struct DzConst {
+    float ka, kb;
+};
+
+template <int FUNC>
+__device__ __forceinline__ DzConst dz_const(const float* z, int D) {
+    DzConst k{0.0f, 0.0f};
+    if constexpr (FUNC == F_ACKLEY) {
+        float s1 = 0.0f, s2 = 0.0f;
+        for (int j = 0; j < D; ++j) { s1 = s1 + z[j] * z[j]; s2 = s2 + sinpi_sq(z[j]); }
+        const float Df = (float)D;
+        const float r = sqrtf(s1 / Df);
+        const float e1 = expm1_f(-0.2f * r) + 1.0f;
+        const float e2 = expm1_f(-2.0f * (s2 / Df)) + 1.0f;
+        k.ka = (r > 0.0f) ? (4.0f * e1) / (Df * r) : 0.0f;
+        k.kb = (6.28318548f * 2.71828175f / Df) * e2;
+    }
+    return k;
+}
+
+// Griewank's factor cos(theta_j) = 1 - 2 sin^2(theta_j / 2), theta_j / (2 pi) = z_j tab[j]
+__device__ __forceinline__ float griewank_cos(float zj, float tj) { return 1.0f - 2.0f * sinpi_sq(zj * tj); }
+
+// d f / d z_j (0 for j >= D) of the point z; Griewank takes pre = prod_{k<j} cos(theta_k)
+// and suf = prod_{k>j} cos(theta_k), accumulated from the top.
+template <int FUNC>
+__device__ __forceinline__ float dz_term(int j, const float* z, int D, const float* tab, const DzConst& kc, float pre,
+                                         float suf) {
+    float v = 0.0f;
+    if (j < D) {
+        const float zj = z[j];
+        if constexpr (FUNC == F_SPHERE) v = 2.0f * zj;
+        else if constexpr (FUNC == F_ELLIPTIC) v = 2.0f * tab[j] * zj;
+        else if constexpr (FUNC == F_RASTRIGIN) v = 2.0f * zj + 62.8318531f * sin2pi(zj);
+        else if constexpr (FUNC == F_ACKLEY) v = kc.ka * zj + kc.kb * sin2pi(zj);
+        else if constexpr (FUNC == F_ROSENBROCK) {
+            if (j + 1 < D) v = 400.0f * zj * (zj * zj - z[j + 1]) + 2.0f * (zj - 1.0f);
+            if (j > 0) v = v - 200.0f * (z[j - 1] * z[j - 1] - zj);
+        } else if constexpr (FUNC == F_GRIEWANK) {
+            const float sn = sin2pi(zj * tab[j]);  // sin(theta_j)
+            v = zj / 2000.0f + pre * suf * sn * (6.28318548f * tab[j]);
+        } else if constexpr (FUNC == F_WEIERSTRASS) {
+            // sum_k 2^(1-k) pi 3^k sin(2 pi 3^k z), fractions of 3^k |z| as in the forward pass
+            const float a = __builtin_fabsf(zj);
+            float acc = 0.0f;
+            if (a < 8388608.0f) {
+                float c = 6.28318548f;  // 2 pi (1.5)^k
+                if (a < 2.32830644e-10f) {
+                    float p3 = 1.0f;
+                    for (int k = 0; k <= 20; ++k) {
+                        acc = fmaf(c, sin2pi(a * p3), acc);
+                        c = c * 1.5f;
+                        p3 = p3 * 3.0f;
+                    }
+                } else {
+                    const uint32_t u = __float_as_uint(a);
+                    const int ex = (int)((u >> 23) & 0xFFu) - 127;
+                    const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
+                    uint64_t y = m << (64 - (23 - ex));
+                    for (int k = 0; k <= 20; ++k) {
+                        const float r = (float)(uint32_t)(y >> 32) * 2.3283064365386963e-10f;
+                        acc = fmaf(c, sin2pi(r), acc);
+                        c = c * 1.5f;
+                        y = y + (y << 1);
+                    }
+                }
+            }
+            v = __builtin_copysignf(1.0f, zj) * acc;
+        } else if constexpr (FUNC == F_SCHWEFEL) {
+            const float sq = sqrtf(__builtin_fabsf(zj));
+            const float t = sq * 0.159154937f;
+            v = -(sin2pi(t) + 0.5f * sq * sin2pi(t + 0.25f));
+        }
+    }
+    return v;
+}
+
 // d f / d z_j of the lane's point for j = 2 s + h (the B-operand layout of the
 // second contraction), from the full z row in LDS.
 template <int KB, int FUNC>
 __device__ __forceinline__ void dfdz(float (&g)[4 * KB], const float* z, float* pre, int D, const float* tab, int h) {
     constexpr int HS = 4 * KB;
-    float s1 = 0.0f, s2 = 0.0f;
-    if constexpr (FUNC == F_ACKLEY) {
-        for (int j = 0; j < D; ++j) { s1 = s1 + z[j] * z[j]; s2 = s2 + sinpi_sq(z[j]); }
-    }
     if constexpr (FUNC == F_GRIEWANK) {
-        // prefix products of cos(theta_k) with cos(theta) = 1 - 2 sin^2(theta/2)
-        float p = 1.0f;
+        float p = 1.0f;  // prefix products
         for (int j = 0; j < D; ++j) {
             pre[j] = p;
-            const float c = 1.0f - 2.0f * sinpi_sq(z[j] * tab[j]);
-            p = p * c;
+            p = p * griewank_cos(z[j], tab[j]);
         }
     }
-    const float Df = (float)D;
-    float ka = 0.0f, kb = 0.0f;
-    if constexpr (FUNC == F_ACKLEY) {
-        const float r = sqrtf(s1 / Df);
-        const float e1 = expm1_f(-0.2f * r) + 1.0f;
-        const float e2 = expm1_f(-2.0f * (s2 / Df)) + 1.0f;
-        ka = (r > 0.0f) ? (4.0f * e1) / (Df * r) : 0.0f;
-        kb = (6.28318548f * 2.71828175f / Df) * e2;
-    }
+    const DzConst kc = dz_const<FUNC>(z, D);
     float suf = 1.0f;  // Griewank suffix product, walked downwards
     int jn = D - 1;
 #pragma unroll
     for (int s = HS - 1; s >= 0; --s) {
         const int j = 2 * s + h;
-        float v = 0.0f;
-        if (j < D) {
-            const float zj = z[j];
-            if constexpr (FUNC == F_SPHERE) v = 2.0f * zj;
-            else if constexpr (FUNC == F_ELLIPTIC) v = 2.0f * tab[j] * zj;
-            else if constexpr (FUNC == F_RASTRIGIN) v = 2.0f * zj + 62.8318531f * sin2pi(zj);
-            else if constexpr (FUNC == F_ACKLEY) v = ka * zj + kb * sin2pi(zj);
-            else if constexpr (FUNC == F_ROSENBROCK) {
-                if (j + 1 < D) v = 400.0f * zj * (zj * zj - z[j + 1]) + 2.0f * (zj - 1.0f);
-                if (j > 0) v = v - 200.0f * (z[j - 1] * z[j - 1] - zj);
-            } else if constexpr (FUNC == F_GRIEWANK) {
-                while (jn > j) {  // suffix = prod_{k > j} cos(theta_k)
-                    suf = suf * (1.0f - 2.0f * sinpi_sq(z[jn] * tab[jn]));
-                    --jn;
-                }
-                const float sn = sin2pi(zj * tab[j]);  // sin(theta_j)
-                v = zj / 2000.0f + pre[j] * suf * sn * (6.28318548f * tab[j]);
-            } else if constexpr (FUNC == F_WEIERSTRASS) {
-                // sum_k 2^(1-k) pi 3^k sin(2 pi 3^k z), fractions of 3^k |z| as in the forward pass
-                const float a = __builtin_fabsf(zj);
-                float acc = 0.0f;
-                if (a < 8388608.0f) {
-                    float c = 6.28318548f;  // 2 pi (1.5)^k
-                    if (a < 2.32830644e-10f) {
-                        float p3 = 1.0f;
-                        for (int k = 0; k <= 20; ++k) {
-                            acc = fmaf(c, sin2pi(a * p3), acc);
-                            c = c * 1.5f;
-                            p3 = p3 * 3.0f;
-                        }
-                    } else {
-                        const uint32_t u = __float_as_uint(a);
-                        const int ex = (int)((u >> 23) & 0xFFu) - 127;
-                        const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
-                        uint64_t y = m << (64 - (23 - ex));
-                        for (int k = 0; k <= 20; ++k) {
-                            const float r = (float)(uint32_t)(y >> 32) * 2.3283064365386963e-10f;
-                            acc = fmaf(c, sin2pi(r), acc);
-                            c = c * 1.5f;
-                            y = y + (y << 1);
-                        }
-                    }
-                }
-                v = __builtin_copysignf(1.0f, zj) * acc;
-            } else if constexpr (FUNC == F_SCHWEFEL) {
-                const float sq = sqrtf(__builtin_fabsf(zj));
-                const float t = sq * 0.159154937f;
-                v = -(sin2pi(t) + 0.5f * sq * sin2pi(t + 0.25f));
+        if constexpr (FUNC == F_GRIEWANK) {
+            while (jn > j) {
+                suf = suf * griewank_cos(z[jn], tab[jn]);
+                --jn;
             }
         }
-        g[s] = v;
+        g[s] = dz_term<FUNC>(j, z, D, tab, kc, (FUNC == F_GRIEWANK && j < D) ? pre[j] : 0.0f, suf);
     }
 }
 
+// Polish staging per elite slot (floats): the point in the [even | odd] row layout, z,
+// dz, the gradient and Griewank's prefix / suffix products (natural order), scalars,
+// and the line search's candidates of one round (fitness, step) -- 4 waves x cpe.
+template <int KB>
+struct PolishSlot {
+    static constexpr int RS = Shape<KB>::RS, ZS = ObjShape<KB>::ZS;
+    __host__ __device__ static constexpr int floats(int cpe) { return RS + 5 * ZS + 16 + 8 * cpe; }
+};
 
-// Orders the LDS traffic of one wave around a cross-lane exchange.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Gradient of the lane's point xrow (its parity half at xrow[h HS ..]): z = Q (x - o)
-// on MFMA staged in zrow, dz = df/dz (pre: Griewank prefix products), gx = Q^T dz
-// on MFMA staged back into zrow in dimension order; returns |gx|^2 summed
-// sequentially over the dimensions.
-template <int KB, int FUNC>
-__device__ __forceinline__ float gradient_tile(const float* xrow, const Geo& g, const Lds& L, float* zrow, float* pre,
-                                               int col, int h) {
-    using S = Shape<KB>;
-    constexpr int RS = S::RS, HS = S::HS, NT = S::NT;
-    {
-        f16v acc[NT];
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
-#pragma unroll
-        for (int s = 0; s < HS; ++s) {
-            const float w = xrow[h * HS + s] - L.o[h * HS + s];
-#pragma unroll
-            for (int jt = 0; jt < NT; ++jt)
-                acc[jt] = __builtin_amdgcn_mfma_f32_32x32x2f32(L.Q[(32 * jt + col) * RS + h * HS + s], w, acc[jt], 0, 0, 0);
-        }
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) zrow[32 * jt + 8 * (r >> 2) + 4 * h + (r & 3)] = acc[jt][r];
-    }
-    wave_sync();
-    float dz[HS];
-    dfdz<KB, FUNC>(dz, zrow, pre, g.D, L.tab, h);
-    f16v gacc[NT];
-#pragma unroll
-    for (int dt = 0; dt < NT; ++dt)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) gacc[dt][r] = 0.0f;
-#pragma unroll
-    for (int s = 0; s < HS; ++s) {
-        const int j = 2 * s + h;
-#pragma unroll
-        for (int dt = 0; dt < NT; ++dt) {
-            const int d = 32 * dt + col;
-            const float qa = (j < 32 * NT) ? L.Q[j * RS + (d & 1) * HS + (d >> 1)] : 0.0f;
-            gacc[dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa, dz[s], gacc[dt], 0, 0, 0);
-        }
-    }
-    wave_sync();
-#pragma unroll
-    for (int dt = 0; dt < NT; ++dt)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) zrow[32 * dt + 8 * (r >> 2) + 4 * h + (r & 3)] = gacc[dt][r];
-    wave_sync();
-    float g2 = 0.0f;
-    for (int d = 0; d < g.D; ++d) g2 = fmaf(zrow[d], zrow[d], g2);
-    return g2;
-}
-
-// Armijo polish of the top-k elites of one task, run between epochs after
-// generation g1 (restates the oracle's polish_task): from a Polyak step
-// alpha = f / |g|^2 along -g, halve alpha until f(clamp(x - alpha g)) <=
-// f - 1e-4 alpha |g|^2 (at most 24 halvings), take the step, repeat.  One lane
-// pair per elite, `polish_waves` waves of 32 elites; the loops are uniform per
-// wave so each MFMA runs with the full wave, and finished lanes keep their point.
+// Armijo polish of the top-k elites of one task, run between epochs after generation g1
+// (restates the oracle's polish_task): from a Polyak step alpha = f / |g|^2 along -g,
+// halve alpha until f(clamp(x - alpha g)) <= f - 1e-4 alpha |g|^2 (at most 24 halvings),
+// take the step, repeat.  The elites go through in groups of 32 / cpe slots, all four
+// waves working on every step of a group: each wave computes one 32-row tile of
+// z = Q (x - o) and of g = Q^T dz for every slot (column c holds slot c / cpe); dz is
+// spread over all threads by (slot, dimension); the line search evaluates 4 x cpe
+// halvings per slot at once (wave w, column sub: halving 4-wave round base + w cpe + sub),
+// and the smallest accepted halving wins -- the step the one-at-a-time search takes.
 // At a transfer boundary the task's elites are published afterwards.
 template <int KB, int FUNC>
 __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, const TaskDev& td) {
     using S = Shape<KB>;
     constexpr int RS = S::RS, HS = S::HS, NT = S::NT, ZS = ObjShape<KB>::ZS;
+    constexpr int NW = kThreads / 64;
     const int tid = threadIdx.x, lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
     Geo g{};
     g.D = td.dim;
@@ -977,7 +946,8 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
     L.tab = L.o + RS;
     L.fit = L.tab + 32 * NT;
     L.ord = reinterpret_cast<int*>(L.fit + g.popP);
-    float* stage = reinterpret_cast<float*>(L.ord + g.popP);
+    const int cpe = a.polish_cpe, nsl = 32 / cpe, PSZ = PolishSlot<KB>::floats(cpe);
+    float* const slots = reinterpret_cast<float*>(L.ord + g.popP);
     const int nz = 32 * NT * RS + RS + 32 * NT;
     for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
     __syncthreads();
@@ -997,90 +967,184 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
     rank_islands(L, g.pop, g.pop, tid);
     __syncthreads();
     const int K = a.p.polish_topk < g.pop ? a.p.polish_topk : g.pop;
-    // cpe columns per elite (a power of two, <= 32): the line search tries cpe step
-    // sizes per round, and the first accepted one in halving order is taken, which
-    // is the step the one-at-a-time search would take.
-    const int cpe = a.polish_cpe, epw = 32 / cpe;  // elites per wave pass
-    const int ntc = (K + epw - 1) / epw, nw = a.polish_waves;
-    if (wave < nw) {
-        const int slot = col / cpe, sub = col - slot * cpe;
-        float* xrow = stage + (3 * epw * wave + slot) * ZS;  // current point, parity halves as in L.X
-        float* zrow = stage + (3 * epw * wave + epw + slot) * ZS;
-        float* pre = stage + (3 * epw * wave + 2 * epw + slot) * ZS;
-        for (int ct = wave; ct < ntc; ct += nw) {
-            const int e = ct * epw + slot;
-            const bool valid = e < K;
-            const int i = L.ord[valid ? e : 0];
-            float* xr = Xg + (int64_t)i * g.D;
-#pragma unroll
-            for (int s = 0; s < HS; ++s) xrow[h * HS + s] = (2 * s + h < g.D) ? xr[2 * s + h] : 0.0f;
-            float f = L.fit[i];
-            bool active = valid;
-            wave_sync();
-            for (int step = 0; step < a.p.polish_steps; ++step) {
-                const float g2 = gradient_tile<KB, FUNC>(xrow, g, L, zrow, pre, col, h);
-                if (!(g2 > 0.0f) || !(f > 0.0f)) active = false;
-                const float alpha0 = f / g2;
-                bool done = false;
-                float fnew = f, alpha_ok = alpha0;
-                float xn[HS];
-                for (int b0 = 0; b0 < 24; b0 += cpe) {
-                    asm volatile("" ::: "memory");  // re-read the staged rows: keeps them out of registers
-                    const int bb = b0 + sub;
-                    float alpha = alpha0;
-                    for (int q = 0; q < bb; ++q) alpha = alpha * 0.5f;
-#pragma unroll
-                    for (int s = 0; s < HS; ++s) {
-                        const float st = alpha * zrow[2 * s + h];
-                        xn[s] = (2 * s + h < g.D) ? clampf(xrow[h * HS + s] - st, td.lo, td.hi) : 0.0f;
-                    }
-                    const float fn = evaluate_tile<KB, FUNC>(xn, g, L, col, h);
-                    const float c1 = 1.0e-4f * alpha;
-                    const float c2 = c1 * g2;
-                    const float thr = f - c2;
-                    const bool ok = active && !done && bb < 24 && fn <= thr;
-                    // first accepting column of this elite's group (lane half 0 carries the vote)
-                    const uint32_t vote = (uint32_t)__ballot(ok);
-                    const uint32_t grp = (vote >> (slot * cpe)) & ((cpe == 32) ? 0xFFFFFFFFu : ((1u << cpe) - 1u));
-                    const int win = grp ? __builtin_ctz(grp) : 0;
-                    const int src = slot * cpe + win;  // both lane halves follow half 0
-                    const float fn_w = __shfl(fn, src);
-                    const float al_w = __shfl(alpha, src);
-                    if (active && !done && grp) {
-                        done = true;
-                        fnew = fn_w;
-                        alpha_ok = al_w;
-                    }
-                    if (!__any(active && !done)) break;
-                }
-#pragma unroll
-                for (int s = 0; s < HS; ++s) {
-                    const float st = alpha_ok * zrow[2 * s + h];
-                    xn[s] = (2 * s + h < g.D) ? clampf(xrow[h * HS + s] - st, td.lo, td.hi) : 0.0f;
-                }
-                wave_sync();
-                if (done) {
-#pragma unroll
-                    for (int s = 0; s < HS; ++s) xrow[h * HS + s] = xn[s];
-                    f = fnew;
-                } else {
-                    active = false;
-                }
-                wave_sync();
-            }
-            if (valid && sub == 0) {
-#pragma unroll
-                for (int s = 0; s < HS; ++s)
-                    if (2 * s + h < g.D) xr[2 * s + h] = xrow[h * HS + s];
-                if (h == 0) {
-                    L.fit[i] = f;
-                    fitg[i] = f;
-                }
-            }
-            wave_sync();
+    // per-slot views
+    auto X = [&](int sl) { return slots + sl * PSZ; };
+    auto Z = [&](int sl) { return X(sl) + RS; };
+    auto DZ = [&](int sl) { return Z(sl) + ZS; };
+    auto G = [&](int sl) { return DZ(sl) + ZS; };
+    auto PRE = [&](int sl) { return G(sl) + ZS; };
+    auto SUF = [&](int sl) { return PRE(sl) + ZS; };
+    auto SF = [&](int sl) { return SUF(sl) + ZS; };  // f, g2, alpha0, ka, kb, fnew, alpha_ok
+    auto SI = [&](int sl) { return reinterpret_cast<int*>(SF(sl) + 8); };  // active, done, first, elite
+    auto CF = [&](int sl) { return SF(sl) + 16; };
+    auto CA = [&](int sl) { return CF(sl) + NW * cpe; };
+    constexpr int kNone = 1 << 30;
+    const int my_sl = col / cpe, sub = col - my_sl * cpe;
+    for (int gb = 0; gb < K; gb += nsl) {
+        const int ns = (K - gb) < nsl ? (K - gb) : nsl;
+        for (int e = tid; e < nsl * 2 * HS; e += kThreads) {
+            const int sl = e / (2 * HS), c = e - sl * (2 * HS);
+            const int d = (c < HS) ? 2 * c : 2 * (c - HS) + 1;
+            float v = 0.0f;
+            if (sl < ns && d < g.D) v = Xg[(int64_t)L.ord[gb + sl] * g.D + d];
+            X(sl)[c] = v;
         }
+        for (int sl = tid; sl < nsl; sl += kThreads) {
+            SF(sl)[0] = sl < ns ? L.fit[L.ord[gb + sl]] : 0.0f;
+            SI(sl)[0] = sl < ns ? 1 : 0;
+            SI(sl)[3] = sl < ns ? L.ord[gb + sl] : 0;
+        }
+        __syncthreads();
+        for (int step = 0; step < a.p.polish_steps; ++step) {
+            // z = Q (x - o) of every slot, one 32-row tile per wave
+            for (int jt = wave; jt < NT; jt += NW) {
+                const float* xr = X(my_sl) + h * HS;
+                f16v acc;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+                for (int s2 = 0; s2 < g.Dh; ++s2) {
+                    const float w = xr[s2] - L.o[h * HS + s2];
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(L.Q[(32 * jt + col) * RS + h * HS + s2], w, acc, 0, 0, 0);
+                }
+                if (sub == 0) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) Z(my_sl)[32 * jt + 8 * (r >> 2) + 4 * h + (r & 3)] = acc[r];
+                }
+            }
+            __syncthreads();
+            // per-slot factors: Ackley's, Griewank's prefix (even threads) / suffix (odd) products
+            if constexpr (FUNC == F_ACKLEY) {
+                for (int sl = tid; sl < ns; sl += kThreads) {
+                    const DzConst kc = dz_const<FUNC>(Z(sl), g.D);
+                    SF(sl)[3] = kc.ka;
+                    SF(sl)[4] = kc.kb;
+                }
+                __syncthreads();
+            }
+            if constexpr (FUNC == F_GRIEWANK) {
+                for (int e = tid; e < 2 * ns; e += kThreads) {
+                    const int sl = e >> 1;
+                    const float* z = Z(sl);
+                    float p = 1.0f;
+                    if ((e & 1) == 0) {
+                        for (int j = 0; j < g.D; ++j) {
+                            PRE(sl)[j] = p;
+                            p = p * griewank_cos(z[j], L.tab[j]);
+                        }
+                    } else {
+                        for (int j = g.D - 1; j >= 0; --j) {
+                            SUF(sl)[j] = p;
+                            p = p * griewank_cos(z[j], L.tab[j]);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            // dz of every (slot, dimension); zero past the dimension and for empty slots
+            for (int e = tid; e < nsl * 2 * HS; e += kThreads) {
+                const int sl = e / (2 * HS), j = e - sl * (2 * HS);
+                float v = 0.0f;
+                if (sl < ns) {
+                    const DzConst kc{SF(sl)[3], SF(sl)[4]};
+                    const bool gw = FUNC == F_GRIEWANK && j < g.D;
+                    v = dz_term<FUNC>(j, Z(sl), g.D, L.tab, kc, gw ? PRE(sl)[j] : 0.0f, gw ? SUF(sl)[j] : 0.0f);
+                }
+                DZ(sl)[j] = v;
+            }
+            __syncthreads();
+            // g = Q^T dz of every slot, one 32-row tile of dimensions per wave
+            for (int dt = wave; dt < NT; dt += NW) {
+                const float* dzr = DZ(my_sl);
+                const int d = 32 * dt + col;
+                f16v gacc;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) gacc[r] = 0.0f;
+                for (int s2 = 0; s2 < g.Dh; ++s2) {
+                    const int j = 2 * s2 + h;
+                    gacc = __builtin_amdgcn_mfma_f32_32x32x2f32(L.Q[j * RS + (d & 1) * HS + (d >> 1)], dzr[j], gacc, 0, 0,
+                                                                0);
+                }
+                if (sub == 0) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) G(my_sl)[32 * dt + 8 * (r >> 2) + 4 * h + (r & 3)] = gacc[r];
+                }
+            }
+            __syncthreads();
+            // |g|^2 summed in dimension order, the Polyak step
+            for (int sl = tid; sl < ns; sl += kThreads) {
+                const float* gr = G(sl);
+                float g2 = 0.0f;
+                for (int d = 0; d < g.D; ++d) g2 = fmaf(gr[d], gr[d], g2);
+                const float f = SF(sl)[0];
+                SF(sl)[1] = g2;
+                SF(sl)[2] = f / g2;
+                if (!(g2 > 0.0f) || !(f > 0.0f)) SI(sl)[0] = 0;
+                SI(sl)[1] = 0;
+                SI(sl)[2] = kNone;
+            }
+            __syncthreads();
+            for (int r0 = 0;; r0 += NW * cpe) {
+                const int b = r0 + wave * cpe + sub;
+                const bool act = my_sl < ns && SI(my_sl)[0] && !SI(my_sl)[1];
+                float alpha = SF(my_sl)[2];
+                for (int q = 0; q < b; ++q) alpha = alpha * 0.5f;
+                const float* xr = X(my_sl) + h * HS;
+                const float* gr = G(my_sl);
+                float xn[HS];
+#pragma unroll
+                for (int s2 = 0; s2 < HS; ++s2) {
+                    const float st = alpha * gr[2 * s2 + h];
+                    xn[s2] = (2 * s2 + h < g.D) ? clampf(xr[s2] - st, td.lo, td.hi) : 0.0f;
+                }
+                const float fn = evaluate_tile<KB, FUNC>(xn, g, L, col, h);
+                const float c1 = 1.0e-4f * alpha;
+                const float c2 = c1 * SF(my_sl)[1];
+                const float thr = SF(my_sl)[0] - c2;
+                if (act && b < 24 && fn <= thr && h == 0) {
+                    CF(my_sl)[b - r0] = fn;
+                    CA(my_sl)[b - r0] = alpha;
+                    atomicMin(&SI(my_sl)[2], b);
+                }
+                __syncthreads();
+                for (int sl = tid; sl < ns; sl += kThreads) {
+                    const int fb = SI(sl)[2];
+                    if (SI(sl)[0] && !SI(sl)[1] && fb != kNone) {
+                        SI(sl)[1] = 1;
+                        SF(sl)[5] = CF(sl)[fb - r0];
+                        SF(sl)[6] = CA(sl)[fb - r0];
+                    }
+                }
+                __syncthreads();
+                bool more = false;
+                for (int sl = 0; sl < ns; ++sl) more = more || (SI(sl)[0] && !SI(sl)[1]);
+                if (!more || r0 + NW * cpe >= 24) break;
+            }
+            // accepted slots step to clamp(x - alpha g), the others stop
+            for (int e = tid; e < ns * 2 * HS; e += kThreads) {
+                const int sl = e / (2 * HS), c = e - sl * (2 * HS);
+                if (SI(sl)[0] && SI(sl)[1]) {
+                    const int d = (c < HS) ? 2 * c : 2 * (c - HS) + 1;
+                    const float st = SF(sl)[6] * G(sl)[d];
+                    X(sl)[c] = (d < g.D) ? clampf(X(sl)[c] - st, td.lo, td.hi) : 0.0f;
+                }
+            }
+            __syncthreads();
+            for (int sl = tid; sl < ns; sl += kThreads) {
+                if (SI(sl)[0] && SI(sl)[1]) SF(sl)[0] = SF(sl)[5];
+                else SI(sl)[0] = 0;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < ns * g.D; e += kThreads) {
+            const int sl = e / g.D, d = e - sl * g.D;
+            Xg[(int64_t)SI(sl)[3] * g.D + d] = X(sl)[(d & 1) * HS + (d >> 1)];
+        }
+        for (int sl = tid; sl < ns; sl += kThreads) {
+            L.fit[SI(sl)[3]] = SF(sl)[0];
+            fitg[SI(sl)[3]] = SF(sl)[0];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (a.publish) {
         rank_islands(L, g.pop, g.pop, tid);
         __syncthreads();
@@ -1898,13 +1962,12 @@ static hipError_t launch_obj_any(const emtf::ObjArgs& a, hipStream_t s) {
 #define EMTF_OBJ_NAME2(a, b) a##b
 #define EMTF_OBJ_NAME(a, b) EMTF_OBJ_NAME2(a, b)
 
-// Polish kernel LDS: Q, o, tab, fit, ord, then three staged rows (point, z /
-// gradient, Griewank prefix products) per elite slot (32 / cpe) of each active wave.
+// Polish kernel LDS: Q, o, tab, fit, ord, then the staging of 32 / cpe elite slots.
 extern "C" int EMTF_OBJ_NAME(emtf_polish_lds_floats_kb, EMTF_KB)(int pop, int waves, int cpe) {
     using S = emtf::Shape<EMTF_KB>;
+    (void)waves;
     const int popP = (pop + 31) & ~31;
-    const int rows = 3 * (32 / cpe) * waves;
-    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * popP + rows * emtf::ObjShape<EMTF_KB>::ZS;
+    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * popP + (32 / cpe) * emtf::PolishSlot<EMTF_KB>::floats(cpe);
 }
 
 extern "C" int EMTF_OBJ_NAME(emtf_launch_polish_kb, EMTF_KB)(const emtf::EpochArgs* args, const int* order, int n,
