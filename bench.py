@@ -22,6 +22,19 @@ sys.path.insert(0, os.path.join(ROOT, "sg-emtf_amd"))
 
 METRIC = "task-generations/s @1 GPU; time-to-1e-6 error, batched+adjoint vs plain DE"
 PEAK_FP32_MFMA_TFLOPS = 157.3
+# LDS: 128 bytes per clock per CU, 256 CUs at 2.4 GHz
+PEAK_LDS_TBS = 256 * 128 * 2.4e9 / 1e12
+
+
+def cpu_model():
+    """The host CPU's model name (the CPU baseline's cores)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -84,6 +97,7 @@ def cpu_baseline(tasks, params, gens, n_tasks):
     dt = time.perf_counter() - t0
     eng.close()
     return {"value": n_tasks * gens / dt, "unit": "task-generations/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{n_tasks} tasks x {gens} generations of the same workload (oracle, C/OpenMP, "
                       f"{threads} threads, {dt:.2f} s)"}
 
@@ -122,12 +136,23 @@ def bench_cvrp(a):
         o.run(g, threads)
         dt = time.perf_counter() - t0
         cpu = {"value": n_inst * g / dt, "unit": "instance-generations/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
                "sample": f"{n_inst} instances x {g} generations (oracle, C/OpenMP, {threads} threads, {dt:.2f} s)"}
+    # algorithmic LDS bytes per offspring (two parents and the child of n one-byte genes,
+    # the split's gathers over routes of about 9 customers, the mutation): about 12 n
+    lds_bytes = sum(12 * ins["n"] * ins["pop"] for ins in inst) * 100
+    achieved = lds_bytes / (ms / steps / 1e3) / 1e12
     out = {"metric": "CVRP instance-generations/s @1 GPU", "value": value, "unit": "instance-generations/s",
            "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
            "config": {"workload": "CVRP GA: 512 instances, n~U{50..200}, capacity 50, pop 128, 4 islands, "
                                   "2 elites every 25 gens", "generations": 100 * steps, "generations_per_step": 100},
+           "roofline": {"bound": "lds", "achieved": achieved, "peak": PEAK_LDS_TBS, "unit": "TB/s",
+                        "frac": achieved / PEAK_LDS_TBS, "traffic": None,
+                        "algorithmic": "12 n bytes of LDS per offspring (parents + child + split gathers), "
+                                       "summed over instances and 100 generations per launch",
+                        "note": "the split is a dependent chain per individual (latency-bound), far below "
+                                "the LDS bandwidth"},
            "cpu_baseline": cpu, "mean_best_cost": float(best.mean())}
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

@@ -2,13 +2,17 @@
 //
 // One workgroup owns one instance for an epoch of generations; one lane owns one
 // individual.  Everything an epoch touches lives in LDS: the giant tours (uint8
-// customer ids), the children, the (n+1)^2 uint8 distance matrix (nint Euclidean
-// distances of integer coordinates are <= 141 on [0,100]^2), demands, and each
-// lane's ring of split labels.  Per generation a lane
+// customer ids), the children, the node coordinates (relative to the depot, int16
+// pairs), the depot distances and demands, and each lane's ring of split values.
+// Distances between customers are computed when needed: nint(sqrt(dx^2 + dy^2)) of
+// integer differences is exact in binary32 (sqrt of an integer <= 65280 lies at least
+// 4.9e-4 from every half-integer, far above the square root's rounding), and keeping
+// no (n+1)^2 matrix lets two instances of 200 customers share a CU.  Per generation a lane
 //   1. draws its mate, OX cut points and mutation from Philox (registers),
 //   2. builds its child with Davis order crossover (OX) + swap / 2-opt mutation,
 //   3. decodes it with the Prins split (shortest path over the giant tour,
-//      routes of load <= capacity) in int32,
+//      routes of load <= capacity) in int32, as a sliding-window minimum over a
+//      per-lane ring of packed (label term, prefix demand) entries,
 //   4. keeps the child iff its cost is <= the parent's (reference Selection rule,
 //      proposed model/SG-EMTF_cvrp.cu).
 // The reference's CVRP kernel (SG-EMTF_cvrp.cu:668-962) is a prototype whose
@@ -18,19 +22,26 @@
 #include "emtf_math.h"
 #include "cvrp_types.h"
 
+#ifndef EMTF_CVRP_SPLIT_BLOCK
+#define EMTF_CVRP_SPLIT_BLOCK 8
+#endif
+
 namespace emtf {
 
 constexpr uint32_t P_CV_INIT = 16, P_CV_GEN = 17;
-constexpr int kRing = 64;  // split labels kept per lane; routes hold < 64 customers
-constexpr int kSplitBlock = 4;  // steps of the split's inner loop per round of loads (4 beat 6 and 8)
+constexpr int kRing = 64;  // ring entries kept per lane; routes hold < 64 customers
+constexpr int kSplitBlock = EMTF_CVRP_SPLIT_BLOCK;  // ring entries loaded per round of the window scan
 
-template <typename LabelT>
+// RingT: uint16_t entries (W + woff) when every instance's W + woff range fits 16 bits,
+// else int32_t (W itself, woff = 0).
+template <typename RingT>
 struct CvLds {
     uint8_t* P;     // popP x NP tours
     uint8_t* C;     // popP x NP children
-    uint8_t* dist;  // (n+1) x DS
-    uint8_t* dem;   // DS
-    LabelT* V;      // kRing x popP split labels (uint16 when every label of the batch fits)
+    uint32_t* xy;   // S node coordinates relative to the depot, (x & 0xFFFF) | (y << 16)
+    uint8_t* d0;    // DS depot distances d(0, c)
+    uint8_t* dem;   // DS demands
+    RingT* R;       // kRing x popP split ring
     uint32_t* used; // 8 x popP OX bitmaps
     int32_t* cost;  // popP
     int32_t* ccost; // popP
@@ -38,65 +49,87 @@ struct CvLds {
     int32_t* aux;   // popP
 };
 
-__host__ __device__ inline int cvrp_lds_bytes(int n, int popP, int label_bytes) {
+__host__ __device__ inline int cvrp_lds_bytes(int n, int popP, int ring_bytes) {
     const int NP = (n + 3) & ~3, DS = (n + 1 + 3) & ~3;
-    return 2 * popP * NP + (n + 1) * DS + DS + kRing * popP * label_bytes + 8 * popP * 4 + 4 * popP * 4;
+    return kRing * popP * ring_bytes + 8 * popP * 4 + 4 * popP * 4 + 4 * DS + 2 * DS + 2 * popP * NP;
 }
 
-// Prins split of the lane's tour t (LDS row); labels in the lane's ring column.
-// The pull loop over route starts i = j-1, j-2, ... runs in blocks of kSplitBlock
-// steps: the block's customer ids are loaded, then their demands, distances and labels, then
-// the steps are applied in order until the load exceeds the capacity -- two rounds
-// of LDS latency per block, with the sums and the minimum formed exactly as one
-// step at a time.  Loads past the stopping step stay in bounds (position < 0 reads
-// the depot; ring slots wrap) and are not used.
-template <typename LabelT>
-__device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvLds<LabelT>& L, int S, int cap, int lane,
-                                              int popP) {
-    L.V[0 * popP + lane] = 0;
+// TSPLIB EUC_2D distance nint(sqrt(dx^2 + dy^2)) of two packed relative coordinates
+__device__ __forceinline__ int nint_xy(uint32_t a, uint32_t b) {
+    const int dx = (int)(int16_t)(a & 0xFFFFu) - (int)(int16_t)(b & 0xFFFFu);
+    const int dy = (int)(int16_t)(a >> 16) - (int)(int16_t)(b >> 16);
+    const float s = (float)(dx * dx + dy * dy);
+    return (int)(__builtin_sqrtf(s) + 0.5f);
+}
+
+// Prins split of the lane's tour t (LDS row) as a sliding-window minimum.  With path
+// prefix sums P_k = sum_{m<k} d(t_m, t_{m+1}), the route t_i .. t_{j-1} costs
+// d(0, t_i) + P_{j-1} - P_i + d(t_{j-1}, 0), so the label of j is
+//   V_j = P_{j-1} + d(t_{j-1}, 0) + min W_i,  W_i = V_i + d(0, t_i) - P_i,
+// over the route starts i in [lo, j) whose demand fits the capacity (V_0 = 0) -- the same
+// integer sums and minimum as the pull form, so the same labels.  The window start lo
+// follows j (the demand of t_lo .. t_{j-1} kept in a register).  The lane's ring holds
+// W_i + woff for the last kRing positions (one slot per position, shared by the wave, so
+// conflict-free); it starts full of the largest value and every entry is overwritten
+// with it when it leaves the window, so the minimum over the newest 16 slots needs no
+// mask (older or stale slots hold the largest value or, aliased by the ring, a value of
+// the window) -- windows longer than 16 continue in blocks of 8.
+template <typename RingT>
+__device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvLds<RingT>& L, int cap, int lane,
+                                              int popP, int woff) {
+    constexpr RingT NONE = (RingT)((sizeof(RingT) == 2) ? 0xFFFF : 0x7FFFFFFF);
+    RingT* R = L.R + lane;
+#pragma unroll
+    for (int k = 0; k < kRing; ++k) R[k * popP] = NONE;
+    int cur = t[0];
+    uint32_t xyc = L.xy[cur];
+    int d0c = L.d0[cur];   // d(0, t_{j-1}) = d(t_{j-1}, 0)
+    int dcur = L.dem[cur];
+    int32_t P = 0;         // P_{j-1}
+    int lo = 0, wload = 0; // window start, demand of t_lo .. t_{j-1}
+    R[0] = (RingT)(d0c + woff);  // W_0 = d(0, t_0)
+    int32_t V = 0;
     for (int j = 1; j <= n; ++j) {
-        int32_t best = 0x7fffffff;
-        int load = 0, path = 0;
-        const int last = t[j - 1];
-        const int back = L.dist[last * S + 0];
-        int prev = last;  // customer at position i + 1 (for i = j - 1 its distance is unused)
-        bool stop = false;
-        for (int i0 = j - 1; i0 >= 0 && !stop; i0 -= kSplitBlock) {
-            int tk[kSplitBlock], dm[kSplitBlock], dd[kSplitBlock], d0[kSplitBlock];
-            int32_t v[kSplitBlock];
-#pragma unroll
-            for (int k = 0; k < kSplitBlock; ++k) tk[k] = (i0 - k >= 0) ? t[i0 - k] : 0;
-#pragma unroll
-            for (int k = 0; k < kSplitBlock; ++k) {
-                const int nx = (k == 0) ? prev : tk[k - 1];
-                dm[k] = L.dem[tk[k]];
-                dd[k] = L.dist[tk[k] * S + nx];
-                d0[k] = L.dist[tk[k]];
-                v[k] = (int32_t)L.V[((i0 - k) & (kRing - 1)) * popP + lane];
-            }
-#pragma unroll
-            for (int k = 0; k < kSplitBlock; ++k) {
-                const int i = i0 - k;
-                if (stop || i < 0) break;
-                load += dm[k];
-                if (load > cap) {
-                    stop = true;
-                    break;
-                }
-                if (i < j - 1) path += dd[k];
-                const int32_t c = v[k] + d0[k] + path + back;
-                best = c < best ? c : best;
-            }
-            prev = tk[kSplitBlock - 1];
+        // the next customer's coordinates, distances and demand do not depend on this label
+        const int nxt = (j < n) ? t[j] : 0;
+        const uint32_t xyn = L.xy[nxt];
+        const int d0n = L.d0[nxt];
+        const int dmn = L.dem[nxt];
+        const int dnx = nint_xy(xyc, xyn);
+        wload += dcur;
+        while (wload > cap) {  // lo <= j - 1: every demand fits the capacity
+            R[(lo & (kRing - 1)) * popP] = NONE;
+            wload -= L.dem[t[lo]];
+            ++lo;
         }
-        L.V[(j & (kRing - 1)) * popP + lane] = (LabelT)best;
+        int32_t m = (int32_t)NONE;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int32_t w = (int32_t)R[((j - 1 - k) & (kRing - 1)) * popP];
+            m = w < m ? w : m;
+        }
+        for (int i0 = j - 17; i0 >= lo; i0 -= 8) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int32_t w = (int32_t)R[((i0 - k) & (kRing - 1)) * popP];
+                m = w < m ? w : m;
+            }
+        }
+        V = P + d0c + (m - woff);
+        if (j == n) break;
+        P += dnx;
+        R[(j & (kRing - 1)) * popP] = (RingT)(V + d0n - P + woff);
+        cur = nxt;
+        xyc = xyn;
+        d0c = d0n;
+        dcur = dmn;
     }
-    return (int32_t)L.V[(n & (kRing - 1)) * popP + lane];
+    return V;
 }
 
-template <typename LabelT>
+template <typename RingT>
 __device__ __forceinline__ void make_child(const uint8_t* p1, const uint8_t* p2, uint8_t* child, int n, int a, int b,
-                                           int mut, int c, int d, const CvLds<LabelT>& L, int lane, int popP) {
+                                           int mut, int c, int d, const CvLds<RingT>& L, int lane, int popP) {
     uint32_t* used = L.used + lane;
 #pragma unroll
     for (int w = 0; w < 8; ++w) used[w * popP] = 0u;
@@ -105,14 +138,29 @@ __device__ __forceinline__ void make_child(const uint8_t* p1, const uint8_t* p2,
         child[k] = (uint8_t)g;
         used[(g >> 5) * popP] |= 1u << (g & 31);
     }
+    // the rest of p2 from b + 1 on, in order, skipping the genes taken from p1: four
+    // genes and their bitmap words per round of loads
     int k = (b + 1 == n) ? 0 : b + 1;
     int src = k;
-    for (int s = 0; s < n; ++s) {
-        const int g = p2[src];
-        src = (src + 1 == n) ? 0 : src + 1;
-        if (used[(g >> 5) * popP] & (1u << (g & 31))) continue;
-        child[k] = (uint8_t)g;
-        k = (k + 1 == n) ? 0 : k + 1;
+    for (int s = 0; s < n; s += 4) {
+        int g[4];
+        uint32_t u[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int sq = src + q;           // < n + 3
+            const int iq = sq < n ? sq : sq - n;
+            g[q] = p2[iq < n ? iq : 0];       // n < 4: a gene past the tour is read in the row, unused
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) u[q] = used[(g[q] >> 5) * popP];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (s + q < n && !((u[q] >> (g[q] & 31)) & 1u)) {
+                child[k] = (uint8_t)g[q];
+                k = (k + 1 == n) ? 0 : k + 1;
+            }
+        }
+        src = src + 4 < n ? src + 4 : src + 4 - n;
     }
     if (mut == 1) {
         const uint8_t tmp = child[c];
@@ -130,8 +178,8 @@ __device__ __forceinline__ void make_child(const uint8_t* p1, const uint8_t* p2,
     }
 }
 
-template <typename LabelT>
-__device__ __forceinline__ void rank_cost(const CvLds<LabelT>& L, int pop, int m, int tid, int nthreads) {
+template <typename RingT>
+__device__ __forceinline__ void rank_cost(const CvLds<RingT>& L, int pop, int m, int tid, int nthreads) {
     for (int i = tid; i < pop; i += nthreads) {
         const int q = i / m, base = q * m, il = i - base;
         const int32_t fi = L.cost[i];
@@ -144,7 +192,7 @@ __device__ __forceinline__ void rank_cost(const CvLds<LabelT>& L, int pop, int m
     }
 }
 
-template <typename LabelT>
+template <typename RingT>
 __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* __restrict__ order) {
     extern __shared__ __align__(16) uint8_t cv_smem[];
     const int inst = order[blockIdx.x];
@@ -152,10 +200,10 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
     const int n = cd.n, cap = cd.cap, pop = cd.pop, S = n + 1;
     const int NP = (n + 3) & ~3, DS = (n + 1 + 3) & ~3;
     const int popP = blockDim.x, tid = threadIdx.x;
-    CvLds<LabelT> L;
+    CvLds<RingT> L;
     uint8_t* p = cv_smem;
-    L.V = reinterpret_cast<LabelT*>(p);
-    p += kRing * popP * sizeof(LabelT);
+    L.R = reinterpret_cast<RingT*>(p);
+    p += kRing * popP * sizeof(RingT);
     L.used = reinterpret_cast<uint32_t*>(p);
     p += 8 * popP * 4;
     L.cost = reinterpret_cast<int32_t*>(p);
@@ -163,23 +211,24 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
     L.ord = L.ccost + popP;
     L.aux = L.ord + popP;
     p += 4 * popP * 4;
+    L.xy = reinterpret_cast<uint32_t*>(p);
+    p += 4 * DS;
+    L.d0 = p;
+    p += DS;
+    L.dem = p;
+    p += DS;
     L.P = p;
     p += popP * NP;
     L.C = p;
-    p += popP * NP;
-    L.dem = p;
-    p += DS;
-    L.dist = p;
-    (void)DS;
     const Rng rng{a.seed_lo, a.seed_hi, (uint32_t)inst};
     const int m = pop / (a.n_islands > 0 ? a.n_islands : 1);
 
-    const uint8_t* dg = a.dist + cd.dist_off;
-    for (int e = tid; e < S * S; e += popP) {
-        const int i = e / S, j = e - i * S;
-        L.dist[i * S + j] = dg[e];
+    for (int e = tid; e < S; e += popP) {
+        const uint32_t c = a.xy[cd.xy_off + e];
+        L.xy[e] = c;
+        L.d0[e] = (uint8_t)nint_xy(0u, c);
+        L.dem[e] = a.dem[cd.dem_off + e];
     }
-    for (int e = tid; e < S; e += popP) L.dem[e] = a.dem[cd.dem_off + e];
     uint8_t* tg = a.tours + cd.tour_off;
     if (!a.init) {
         for (int e = tid; e < pop * n; e += popP) {
@@ -201,7 +250,7 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
             tour[k] = tour[j];
             tour[j] = tmp;
         }
-        L.cost[tid] = split_cost(tour, n, L, S, cap, tid, popP);
+        L.cost[tid] = split_cost(tour, n, L, cap, tid, popP, cd.woff);
     }
     __syncthreads();
 
@@ -220,7 +269,7 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
             const int c = (int)(b1.x % (uint32_t)n), d = (int)(b1.y % (uint32_t)n);
             uint8_t* child = L.C + tid * NP;
             make_child(L.P + tid * NP, L.P + (base + r) * NP, child, n, ca, cb, mut, c, d, L, tid, popP);
-            L.ccost[tid] = split_cost(child, n, L, S, cap, tid, popP);
+            L.ccost[tid] = split_cost(child, n, L, cap, tid, popP, cd.woff);
         }
         __syncthreads();
         if (tid < pop && L.ccost[tid] <= L.cost[tid]) {
@@ -267,16 +316,16 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
 
 }  // namespace emtf
 
-extern "C" int emtf_cvrp_lds_bytes(int n, int popP, int label_bytes) {
-    return emtf::cvrp_lds_bytes(n, popP, label_bytes);
+extern "C" int emtf_cvrp_lds_bytes(int n, int popP, int ring_bytes) {
+    return emtf::cvrp_lds_bytes(n, popP, ring_bytes);
 }
 
-// label_bytes 2: split labels stored as uint16 (the host checks that every label of the
-// batch, at most the cost of serving each customer alone, is below 2^16); else int32.
+// ring_bytes 2: W + woff in 16 bits (the host checks that every instance's W + woff range
+// fits); else W in 32 bits.
 extern "C" int emtf_cvrp_launch(const emtf::CvrpArgs* args, const int* order, int n_inst, int threads, int lds_bytes,
-                                int label_bytes, hipStream_t stream) {
+                                int ring_bytes, hipStream_t stream) {
     static bool lds_ok[2] = {false, false};
-    const int w = label_bytes == 2 ? 0 : 1;
+    const int w = ring_bytes == 2 ? 0 : 1;
     const void* k = w == 0 ? reinterpret_cast<const void*>(&emtf::cvrp_epoch_kernel<uint16_t>)
                            : reinterpret_cast<const void*>(&emtf::cvrp_epoch_kernel<int32_t>);
     if (!lds_ok[w]) {

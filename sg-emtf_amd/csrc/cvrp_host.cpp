@@ -12,8 +12,8 @@
 #include "cvrp_types.h"
 
 extern "C" int emtf_cvrp_launch(const emtf::CvrpArgs* args, const int* order, int n_inst, int threads, int lds_bytes,
-                                int label_bytes, hipStream_t stream);
-extern "C" int emtf_cvrp_lds_bytes(int n, int popP, int label_bytes);
+                                int ring_bytes, hipStream_t stream);
+extern "C" int emtf_cvrp_lds_bytes(int n, int popP, int ring_bytes);
 extern "C" void emtf_set_error(const char* msg);
 
 namespace {
@@ -32,7 +32,7 @@ struct CvGroup {
 };
 
 struct emtf_cvrp_handle {
-    int T = 0, gen = 0, threads = 64, label_bytes = 4;
+    int T = 0, gen = 0, threads = 64, ring_bytes = 2;  // split ring entry size (2 or 4)
     std::vector<CvGroup> groups;
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev_join;
@@ -44,7 +44,7 @@ struct emtf_cvrp_handle {
     int* d_order = nullptr;
     uint8_t* d_tours = nullptr;
     int32_t* d_cost = nullptr;
-    uint8_t* d_dist = nullptr;
+    uint32_t* d_xy = nullptr;
     uint8_t* d_dem = nullptr;
     hipStream_t stream = nullptr;
 };
@@ -55,7 +55,7 @@ static void cv_free(emtf_cvrp_handle* h) {
     (void)hipFree(h->d_order);
     (void)hipFree(h->d_tours);
     (void)hipFree(h->d_cost);
-    (void)hipFree(h->d_dist);
+    (void)hipFree(h->d_xy);
     (void)hipFree(h->d_dem);
     for (hipStream_t s : h->side) (void)hipStreamDestroy(s);
     for (hipEvent_t e : h->ev_join) (void)hipEventDestroy(e);
@@ -82,7 +82,7 @@ static int cv_launch(emtf_cvrp_handle* h, int g0, int g1, int init, hipStream_t 
     }
     for (const CvGroup& g : h->groups) {
         hipStream_t q = g.stream == 0 ? s : h->side[g.stream - 1];
-        const int rc = emtf_cvrp_launch(&h->a, h->d_order + g.offset, g.count, h->threads, g.lds, h->label_bytes, q);
+        const int rc = emtf_cvrp_launch(&h->a, h->d_order + g.offset, g.count, h->threads, g.lds, h->ring_bytes, q);
         if (rc != 0) return cfail(EMTF_E_DEVICE, std::string("cvrp kernel launch: ") + hipGetErrorString((hipError_t)rc));
     }
     if (fork) {
@@ -122,7 +122,7 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
     h->T = n_inst;
     h->threads = (maxpop + 63) & ~63;
     h->inst.resize(n_inst);
-    int64_t nt = 0, nc = 0, nd = 0, ne = 0;
+    int64_t nt = 0, nc = 0, nd = 0, nx = 0, ne = 0;
     for (int t = 0; t < n_inst; ++t) {
         emtf::CvrpDev& d = h->inst[t];
         std::memset(&d, 0, sizeof d);
@@ -131,40 +131,55 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
         d.pop = in[t].pop;
         d.tour_off = nt;
         d.cost_off = nc;
-        d.dist_off = nd;
+        d.xy_off = nx;
         d.dem_off = ne;
         nt += (int64_t)d.pop * d.n;
         nc += d.pop;
         nd += (int64_t)(d.n + 1) * (d.n + 1);
+        nx += d.n + 1;
         ne += d.n + 1;
     }
+    // the distance matrix stays on the host (checks and bounds); the kernel gets the
+    // coordinates relative to the depot and computes the same nint distances
     std::vector<uint8_t> dist((size_t)nd), dem((size_t)ne);
-    for (int t = 0; t < n_inst; ++t) {
+    std::vector<uint32_t> xyr((size_t)nx);
+    for (int64_t t = 0, off = 0; t < n_inst; ++t) {
         const emtf_cvrp_instance& c = in[t];
         const int S = c.n + 1;
         for (int i = 0; i < S; ++i) {
             dem[h->inst[t].dem_off + i] = (uint8_t)c.demand[i];
+            const int rx = c.xy[2 * i] - c.xy[0], ry = c.xy[2 * i + 1] - c.xy[1];
+            xyr[h->inst[t].xy_off + i] = ((uint32_t)rx & 0xFFFFu) | ((uint32_t)ry << 16);
             for (int j = 0; j < S; ++j) {
                 const int dv = nint_dist(c.xy[2 * i], c.xy[2 * i + 1], c.xy[2 * j], c.xy[2 * j + 1]);
                 if (dv > 255) { cv_free(h); return cfail(EMTF_E_UNSUPPORTED, "distance above 255"); }
-                dist[h->inst[t].dist_off + (int64_t)i * S + j] = (uint8_t)dv;
+                dist[off + (int64_t)i * S + j] = (uint8_t)dv;
             }
         }
+        off += (int64_t)S * S;
     }
-    // 16-bit split labels when every label of the batch fits: a label never exceeds the
-    // cost of serving each customer of the prefix alone, sum_c 2 d(0, c)
-    h->label_bytes = 2;
-    for (int t = 0; t < n_inst && h->label_bytes == 2; ++t) {
+    // Split ring entries W_i + woff with W_i = V_i + d(0, t_i) - P_i: a label V_i never
+    // exceeds the cost of serving each customer alone, sum_c 2 d(0, c), and a path prefix
+    // P_i never exceeds (n - 1) dmax, so woff = (n - 1) dmax makes every entry non-negative
+    // and bounds it by woff + sum_c 2 d(0, c) + dmax.  The ring holds 16-bit entries when
+    // that bound fits for every instance, else W itself in 32 bits (woff = 0).
+    h->ring_bytes = 2;
+    std::vector<int64_t> woffs(n_inst);
+    for (int64_t t = 0, off = 0; t < n_inst; ++t) {
         const int S = in[t].n + 1;
-        int64_t bound = 0;
-        for (int c = 1; c < S; ++c) bound += 2 * (int64_t)dist[h->inst[t].dist_off + c];
-        if (bound >= 65536) h->label_bytes = 4;
+        int64_t vbound = 0, dmax = 0;
+        for (int c = 1; c < S; ++c) vbound += 2 * (int64_t)dist[off + c];
+        for (int64_t e = 0; e < (int64_t)S * S; ++e) dmax = std::max<int64_t>(dmax, dist[off + e]);
+        woffs[t] = (int64_t)(in[t].n - 1) * dmax;
+        if (woffs[t] + vbound + dmax >= 65536) h->ring_bytes = 4;
+        off += (int64_t)S * S;
     }
+    for (int t = 0; t < n_inst; ++t) h->inst[t].woff = h->ring_bytes == 2 ? (int32_t)woffs[t] : 0;
     // groups of equal occupancy (workgroups per CU from the LDS footprint), longest
     // instances first inside each group
     std::vector<int> occ(n_inst);
     for (int t = 0; t < n_inst; ++t) {
-        const int b = emtf_cvrp_lds_bytes(in[t].n, h->threads, h->label_bytes);
+        const int b = emtf_cvrp_lds_bytes(in[t].n, h->threads, h->ring_bytes);
         if (b > kMaxLds) { cv_free(h); return cfail(EMTF_E_UNSUPPORTED, "instance exceeds LDS"); }
         occ[t] = std::min(8, kMaxLds / b);
     }
@@ -178,7 +193,7 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
         CvGroup g;
         g.offset = r;
         while (r < n_inst && occ[order[r]] == occ[order[g.offset]]) {
-            g.lds = std::max(g.lds, emtf_cvrp_lds_bytes(in[order[r]].n, h->threads, h->label_bytes));
+            g.lds = std::max(g.lds, emtf_cvrp_lds_bytes(in[order[r]].n, h->threads, h->ring_bytes));
             ++r;
         }
         g.count = r - g.offset;
@@ -204,13 +219,13 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
     auto ok = [&](hipError_t e) { return e == hipSuccess; };
     if (!ok(hipMalloc((void**)&h->d_inst, sizeof(emtf::CvrpDev) * n_inst)) || !ok(hipMalloc((void**)&h->d_order, sizeof(int) * n_inst)) ||
         !ok(hipMalloc((void**)&h->d_tours, (size_t)nt)) || !ok(hipMalloc((void**)&h->d_cost, sizeof(int32_t) * (size_t)nc)) ||
-        !ok(hipMalloc((void**)&h->d_dist, (size_t)nd)) || !ok(hipMalloc((void**)&h->d_dem, (size_t)ne))) {
+        !ok(hipMalloc((void**)&h->d_xy, sizeof(uint32_t) * (size_t)nx)) || !ok(hipMalloc((void**)&h->d_dem, (size_t)ne))) {
         cv_free(h);
         return cfail(EMTF_E_NOMEM, "hipMalloc failed");
     }
     if (!ok(hipMemcpy(h->d_inst, h->inst.data(), sizeof(emtf::CvrpDev) * n_inst, hipMemcpyHostToDevice)) ||
         !ok(hipMemcpy(h->d_order, order.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice)) ||
-        !ok(hipMemcpy(h->d_dist, dist.data(), (size_t)nd, hipMemcpyHostToDevice)) ||
+        !ok(hipMemcpy(h->d_xy, xyr.data(), sizeof(uint32_t) * (size_t)nx, hipMemcpyHostToDevice)) ||
         !ok(hipMemcpy(h->d_dem, dem.data(), (size_t)ne, hipMemcpyHostToDevice))) {
         cv_free(h);
         return cfail(EMTF_E_DEVICE, "hipMemcpy failed");
@@ -220,7 +235,7 @@ int emtf_cvrp_create(const emtf_cvrp_instance* in, int32_t n_inst, const emtf_cv
     a.n_inst = n_inst;
     a.tours = h->d_tours;
     a.cost = h->d_cost;
-    a.dist = h->d_dist;
+    a.xy = h->d_xy;
     a.dem = h->d_dem;
     a.p_swap = prm->p_swap;
     a.p_2opt = prm->p_2opt;

@@ -131,8 +131,8 @@ def test_gpu_cvrp_bit_exact(oracle_mod, n_islands):
 
 @pytest.mark.gpu
 def test_gpu_cvrp_bit_exact_32bit_labels(oracle_mod):
-    """Instances whose split labels can exceed 2^16 (corner depot, spread coordinates)
-    run with 32-bit labels; results stay bit-exact."""
+    """Instances whose split ring values can exceed 2^16 (corner depot, spread coordinates)
+    run with 32 + 32-bit ring entries; results stay bit-exact."""
     from emtf.cvrp import CvrpEngine
     rng = np.random.default_rng(5)
     inst = []
@@ -147,6 +147,29 @@ def test_gpu_cvrp_bit_exact_32bit_labels(oracle_mod):
     orc = oracle_mod.CvrpEngine(inst, **params)
     gpu.run(9)
     orc.run(9)
+    for i in range(len(inst)):
+        tg, cg = gpu.population(i)
+        to, co = orc.population(i)
+        assert np.array_equal(cg, co), i
+        assert np.array_equal(tg, to), i
+
+
+@pytest.mark.gpu
+def test_gpu_cvrp_long_routes(oracle_mod):
+    """Unit demands and capacity 60: routes (the split's windows) of up to 60 customers,
+    past the split's fixed 16-entry scan; bit-exact with the oracle."""
+    from emtf.cvrp import CvrpEngine
+    rng = np.random.default_rng(8)
+    inst = []
+    for n in (255, 130, 61, 7):
+        xy = np.vstack([[50, 50], rng.integers(0, 101, (n, 2))]).astype(np.int32)
+        inst.append(dict(n=n, capacity=60, pop=64, xy=xy, demand=np.concatenate([[0], np.ones(n)]).astype(np.int32)))
+    params = dict(p_swap=0.3, p_2opt=0.3, n_islands=2, migrate_interval=3, migrate_count=2, seed=4)
+    print("case: cvrp long routes", flush=True)
+    gpu = CvrpEngine(inst, **params)
+    orc = oracle_mod.CvrpEngine(inst, **params)
+    gpu.run(11)
+    orc.run(11)
     for i in range(len(inst)):
         tg, cg = gpu.population(i)
         to, co = orc.population(i)
