@@ -158,7 +158,9 @@ static const float P3F[21] = {1.f, 3.f, 9.f, 27.f, 81.f, 243.f, 729.f, 2187.f, 6
 
 /* Fractional parts of 3^k |z| for 2^-32 <= |z| < 2^23 are carried as 0.64
  * fixed-point words y (frac = y / 2^64), so 3^k |z| mod 1 is y_{k+1} = 3 y_k mod 2^64
- * exactly; each term reads the top 32 bits (correctly rounded to binary32). */
+ * exactly; each term reads the top 32 bits as a signed word q, t = q 2^-32 in
+ * [-1/2, 1/2) (q correctly rounded to binary32): t = frac or frac - 1, so
+ * sin^2(pi t) = sin^2(pi frac), and sinpi_half is odd, so sp * sp needs no |t|. */
 float or_weier_term(float z) {
     if (z != z) return z;
     float a = fabsf(z);
@@ -178,9 +180,9 @@ float or_weier_term(float z) {
     int s = 23 - ex;                            /* fraction bits, 1..55 */
     uint64_t y = m << (64 - s);                 /* integer part shifted out */
     for (int k = 0; k <= 20; ++k) {
-        uint32_t q = (uint32_t)(y >> 32);
-        float r = (float)q * 2.3283064365386963e-10f;
-        float sp = sinpi_half(fminf(r, 1.0f - r));
+        int32_t q = (int32_t)(uint32_t)(y >> 32);
+        float t = (float)q * 2.3283064365386963e-10f;
+        float sp = sinpi_half(t);
         acc = fmaf(pow2i(1 - k), sp * sp, acc);
         y = y + (y << 1);
     }

@@ -190,7 +190,9 @@ __device__ __forceinline__ float pow_f(float x, float y) {
 
 // Weierstrass term sum_{k=0}^{20} 2^(1-k) sin^2(pi 3^k z).  For 2^-32 <= |z| < 2^23
 // the fractional part of 3^k |z| is carried exactly as a 0.64 fixed-point word
-// (3x mod 1 is one 64-bit shift-add); each term reads its top 32 bits.
+// (3x mod 1 is one 64-bit shift-add); each term reads its top 32 bits as a signed
+// word: t = q 2^-32 in [-1/2, 1/2) differs from the fraction by 0 or 1, so
+// sin^2(pi t) is the term, and sinpi_half is odd, so its square needs no |t|.
 __device__ __forceinline__ float weier_term(float z) {
     if (z != z) return z;
     const float a = __builtin_fabsf(z);
@@ -216,11 +218,12 @@ __device__ __forceinline__ float weier_term(float z) {
     uint64_t y = m << (64 - s);
 #pragma unroll
     for (int k = 0; k <= 20; ++k) {
-        // keep the conversion 32-bit: without the barrier the backend widens it to u64->f32
-        uint32_t q = (uint32_t)(y >> 32);
-        asm("" : "+v"(q));
-        const float r = (float)q * 2.3283064365386963e-10f;
-        const float sp = sinpi_half(__builtin_fminf(r, 1.0f - r));
+        // the conversion of the high word as one 32-bit instruction (left to itself the
+        // backend widens it to a 64-bit conversion)
+        float qf;
+        asm("v_cvt_f32_i32 %0, %1" : "=v"(qf) : "v"((uint32_t)(y >> 32)));
+        const float t = qf * 2.3283064365386963e-10f;
+        const float sp = sinpi_half(t);
         acc = fmaf(pow2i(1 - k), sp * sp, acc);
         // y <- 3 y mod 2^64 in one VALU op (the backend otherwise emits two 32-bit
         // multiplies and register moves per step)
