@@ -69,11 +69,15 @@ typedef struct emtf_params {
     int32_t polish_topk;        /* elites polished per task */
     int32_t schedule;           /* EMTF_SCHEDULE_*; no reference counterpart */
     int32_t tune;               /* 0: autotune kernel variants and stream balance from measured
-                                   epoch times (results are the same bits either way); 1: off;
+                                   epoch times (results are the same bits either way; the
+                                   decisions are kept for later creates of a batch with the same
+                                   launch units: in memory for the process, and in the file the
+                                   environment variable EMTF_TUNE_CACHE names, if set); 1: off;
                                    2: the first alternative variant where one exists (the wide
                                    kernel, or its 512-thread build); 3: the wide kernel with Q in
                                    LDS where it fits (2 and 3 for testing) */
     int32_t formulas;           /* EMTF_FORMULAS_*: which formulas evaluate the objectives */
+    int32_t admission;          /* EMTF_ADMIT_*: when tasks start */
 } emtf_params;
 
 /* Objective formulas.  TEXTBOOK: the functions in cancellation-free binary32 forms with
@@ -97,6 +101,16 @@ enum { EMTF_FORMULAS_TEXTBOOK = 0, EMTF_FORMULAS_REFERENCE_DE = 1, EMTF_FORMULAS
  * reserved and returns EMTF_E_UNSUPPORTED at create in this build. */
 enum { EMTF_SCHEDULE_AUTO = 0, EMTF_SCHEDULE_EPOCHS = 1, EMTF_SCHEDULE_PERSISTENT = 2 };
 
+/* When tasks start.  AT_CREATE: every task starts at emtf_create.  ONLINE: none does;
+ * emtf_admit starts batches of them while earlier ones run (the reference's online driver,
+ * SG-EMTF.cu:1842-1958).  The handle's generation counter is then a clock: an admitted task
+ * runs `generations` generations from the clock value at its admission, with its own
+ * generation counter (the one its draws are addressed by) starting at 0, and transfers draw
+ * their donor among the tasks that ran up to the transfer boundary and run on after it --
+ * across batches, as the reference draws among every live task (SG-EMTF.cu:1682-1699).
+ * ONLINE runs the epochs directly (no graph replay), without autotuning and without polish. */
+enum { EMTF_ADMIT_AT_CREATE = 0, EMTF_ADMIT_ONLINE = 1 };
+
 typedef struct emtf_handle emtf_handle;
 
 /* Allocate device state, create generation 0 and evaluate it.
@@ -108,6 +122,13 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
  * after that call's work).  Replaces iter()'s loop of GeneticOpera launches and
  * the knowledgeTransfer handshake (SG-EMTF.cu:1646-1745, kernel :1289). */
 int emtf_run(emtf_handle* h, int32_t generations, void* stream);
+
+/* ONLINE admission: start tasks first .. first + count - 1 (not admitted before) at the
+ * handle's clock, which must be a transfer boundary when transfer is on; they are created
+ * and evaluated on `stream` and run the next `generations` generations of the clock.
+ * Replaces the per-task host thread start of the reference's online driver
+ * (SG-EMTF.cu:1646-1660, :1890-1897). */
+int emtf_admit(emtf_handle* h, int32_t first, int32_t count, int32_t generations, void* stream);
 
 /* Wait for the work queued by emtf_run. */
 int emtf_sync(emtf_handle* h);

@@ -121,6 +121,15 @@ void or_grad64(const or_task* t, int32_t n, const double* x, double* g);
 /* ---- Engine -------------------------------------------------------------- */
 typedef struct or_engine or_engine;
 or_engine* or_create(const or_task* tasks, int32_t n_tasks, const or_params* p);
+/* Online admission: or_create_deferred starts no task; or_admit starts tasks first ..
+ * first + count - 1 at the engine's clock (a transfer boundary) for `generations`
+ * generations, each with its own generation counter from 0.  At a transfer boundary the
+ * donors are the tasks that ran up to it and run on after it. */
+or_engine* or_create_deferred(const or_task* tasks, int32_t n_tasks, const or_params* p);
+int32_t or_admit(or_engine* e, int32_t first, int32_t count, int32_t generations);
+/* The donor `task` draws at the transfer boundary `clock` under the current admissions
+ * (-1: it receives nothing there). */
+int32_t or_transfer_donor(const or_engine* e, int32_t task, int32_t clock);
 void or_destroy(or_engine* e);
 /* Advance every task by n generations.  n_threads <= 0: OpenMP default. */
 void or_run(or_engine* e, int32_t n_gens, int32_t n_threads);

@@ -49,6 +49,10 @@
 #define MUT_XFER_RD(rd, rs) (OR_MUTANT == 5 ? (rs) : (rd))                    /* box map ranges swapped */
 #define MUT_XFER_RS(rd, rs) (OR_MUTANT == 5 ? (rd) : (rs))
 #define MUT_XFER_TGT(P, k, j) (OR_MUTANT == 6 ? (j) : (P) - (k) + (j))         /* replaced rank */
+#define MUT_DONOR_END(end, g) (OR_MUTANT == 8 ? (end) >= (g) : (end) > (g))     /* done at g still a donor */
+#define MUT_DONOR_BASE(base, g) (OR_MUTANT == 9 ? (base) <= (g) : (base) < (g)) /* new at g already a donor */
+#define MUT_DONOR_POS(pos, ti) (OR_MUTANT == 10 ? (ti) : (pos))                 /* skip self by id */
+#define MUT_LOCAL_GEN(g, base) (OR_MUTANT == 11 ? (g) : (g) - (base))           /* draws by the clock */
 
 typedef struct {
     or_task t;          /* pointers below are owned copies */
@@ -66,13 +70,19 @@ typedef struct {
     float* snapF;       /* pop */
     int* order;         /* pop */
     float* elites;      /* transfer_count * D (snapshot for transfer) */
+    /* life on the engine clock: runs generations base + 1 .. end, its own generation
+     * counter (the one its draws are addressed by) is the clock minus base */
+    int base, end;
 } task_state;
+
+#define OR_NEVER 0x7FFFFFFF
 
 struct or_engine {
     int T;
     or_params p;
     int gen;
     task_state* ts;
+    int* donors;        /* scratch: the donor list of a transfer event */
 };
 
 static inline float uniform01(uint32_t w) { return (float)((w >> 8) + 1u) * 5.9604644775390625e-08f; }
@@ -451,12 +461,30 @@ static void snapshot_elites(or_engine* e, int ti) {
         memcpy(s->elites + (size_t)j * D, s->X + (size_t)s->order[j] * D, sizeof(float) * (size_t)D);
 }
 
-static void transfer_task(or_engine* e, int ti, int gen, float* work) {
+/* Transfer into task ti at its generation gen from a donor drawn among the nd tasks of
+ * donors[] (ascending ids, ti among them at position pos): the k-th of the others, k = w mod
+ * (nd - 1) -- the reference draws rand() % T over every task but itself (SG-EMTF.cu:1696-1699),
+ * and over the tasks still running when they arrive online (:1682-1686). */
+/* The donors of the transfer boundary g: the tasks that ran up to it and run on after it
+ * (every task of an engine whose tasks all started at 0), ascending ids; returns their count. */
+static int donor_list(const or_engine* e, int g, int* out) {
+    int nd = 0;
+    for (int ti = 0; ti < e->T; ++ti)
+        if (MUT_DONOR_BASE(e->ts[ti].base, g) && MUT_DONOR_END(e->ts[ti].end, g)) out[nd++] = ti;
+    return nd;
+}
+
+/* The index, in donors[0 .. nd), of the donor that the task at position pos draws with w. */
+static int donor_pick(uint32_t w, int nd, int pos) {
+    int kk = (int)(w % (uint32_t)(nd - 1));
+    return kk + (kk >= pos ? 1 : 0);
+}
+
+static void transfer_task(or_engine* e, int ti, int gen, const int* donors, int nd, int pos, float* work) {
     task_state* s = &e->ts[ti];
-    const int D = s->t.dim, P = s->t.pop, k = e->p.transfer_count, T = e->T;
+    const int D = s->t.dim, P = s->t.pop, k = e->p.transfer_count;
     uint32_t w = draw_word(e->p.seed, (uint32_t)ti, (uint32_t)gen, OR_P_MIG, 0u, 0u, 0u);
-    int dn = (int)(w % (uint32_t)(T - 1));
-    if (dn >= ti) dn += 1;
+    const int dn = donors[donor_pick(w, nd, pos)];
     const task_state* ds = &e->ts[dn];
     const int Dd = ds->t.dim;
     const int same_box = (ds->t.lo == s->t.lo && ds->t.hi == s->t.hi);
@@ -507,9 +535,10 @@ static float* dupf(const float* src, size_t n) {
     return d;
 }
 
-or_engine* or_create(const or_task* tasks, int32_t n_tasks, const or_params* p) {
+static or_engine* create(const or_task* tasks, int32_t n_tasks, const or_params* p, int deferred) {
     or_engine* e = (or_engine*)calloc(1, sizeof(or_engine));
     e->T = n_tasks;
+    e->donors = (int*)calloc((size_t)n_tasks, sizeof(int));
     e->p = *p;
     if (e->p.n_islands < 1) e->p.n_islands = 1;
     e->gen = 0;
@@ -537,10 +566,41 @@ or_engine* or_create(const or_task* tasks, int32_t n_tasks, const or_params* p) 
         s->snapF = dupf(NULL, (size_t)P);
         s->order = (int*)calloc((size_t)P, sizeof(int));
         s->elites = dupf(NULL, (size_t)(e->p.transfer_count > 0 ? e->p.transfer_count : 1) * D);
+        s->base = deferred ? OR_NEVER : 0;
+        s->end = OR_NEVER;
+        if (deferred) /* read back before admission: zero rows, NaN fitness (all bits set) */
+            memset(s->fit, 0xFF, sizeof(float) * (size_t)P);
+    }
+    if (!deferred) {
+#pragma omp parallel for schedule(dynamic)
+        for (int ti = 0; ti < n_tasks; ++ti) init_task(e, ti);
+    }
+    return e;
+}
+
+or_engine* or_create(const or_task* tasks, int32_t n_tasks, const or_params* p) { return create(tasks, n_tasks, p, 0); }
+
+/* Online admission: no task runs until or_admit starts it. */
+or_engine* or_create_deferred(const or_task* tasks, int32_t n_tasks, const or_params* p) {
+    return create(tasks, n_tasks, p, 1);
+}
+
+/* Start tasks first .. first + count - 1 at the current clock for `generations` generations
+ * (their initial population is drawn and evaluated now).  Returns 0, or -1 if a task was
+ * admitted before, the range is bad, or the clock is not on a transfer boundary. */
+int32_t or_admit(or_engine* e, int32_t first, int32_t count, int32_t generations) {
+    const int TI = e->p.transfer_interval;
+    if (first < 0 || count < 1 || first + count > e->T || generations < 0) return -1;
+    if (TI > 0 && e->gen % TI != 0) return -1;
+    for (int ti = first; ti < first + count; ++ti)
+        if (e->ts[ti].base != OR_NEVER) return -1;
+    for (int ti = first; ti < first + count; ++ti) {
+        e->ts[ti].base = e->gen;
+        e->ts[ti].end = e->gen + generations;
     }
 #pragma omp parallel for schedule(dynamic)
-    for (int ti = 0; ti < n_tasks; ++ti) init_task(e, ti);
-    return e;
+    for (int ti = first; ti < first + count; ++ti) init_task(e, ti);
+    return 0;
 }
 
 void or_destroy(or_engine* e) {
@@ -551,6 +611,7 @@ void or_destroy(or_engine* e) {
         free(s->trial); free(s->tfit); free(s->snapX); free(s->snapF); free(s->order); free(s->elites);
     }
     free(e->ts);
+    free(e->donors);
     free(e);
 }
 
@@ -562,28 +623,37 @@ void or_run(or_engine* e, int32_t n_gens, int32_t n_threads) {
 #endif
     const int g_end = e->gen + n_gens;
     const int TI = e->p.transfer_interval;
-    const int xfer_on = (TI > 0 && e->T > 1 && e->p.transfer_count > 0);
+    const int xfer_on = (TI > 0 && e->p.transfer_count > 0);
     while (e->gen < g_end) {
         const int g = e->gen;
-        if (xfer_on && g > 0 && g % TI == 0) {
+        if (xfer_on && g % TI == 0) {
+            const int nd = donor_list(e, g, e->donors);
+            if (nd >= 2) {
 #pragma omp parallel for schedule(dynamic)
-            for (int ti = 0; ti < e->T; ++ti) snapshot_elites(e, ti);
+                for (int q = 0; q < nd; ++q) snapshot_elites(e, e->donors[q]);
 #pragma omp parallel for schedule(dynamic)
-            for (int ti = 0; ti < e->T; ++ti) {
-                float* work = (float*)malloc(sizeof(float) * 2 * (size_t)e->ts[ti].t.dim);
-                transfer_task(e, ti, g, work);
-                free(work);
+                for (int q = 0; q < nd; ++q) {
+                    const int ti = e->donors[q];
+                    float* work = (float*)malloc(sizeof(float) * 2 * (size_t)e->ts[ti].t.dim);
+                    transfer_task(e, ti, MUT_LOCAL_GEN(g, e->ts[ti].base), e->donors, nd, MUT_DONOR_POS(q, ti), work);
+                    free(work);
+                }
             }
         }
+        /* epochs end at transfer boundaries and where a running task ends */
         int g_stop = g_end;
         if (xfer_on) {
             int nb = (g / TI + 1) * TI;
             if (nb < g_stop) g_stop = nb;
         }
+        for (int ti = 0; ti < e->T; ++ti)
+            if (e->ts[ti].base <= g && e->ts[ti].end > g && e->ts[ti].end < g_stop) g_stop = e->ts[ti].end;
 #pragma omp parallel for schedule(dynamic)
         for (int ti = 0; ti < e->T; ++ti) {
-            float* work = (float*)malloc(sizeof(float) * 2 * (size_t)e->ts[ti].t.dim);
-            for (int gg = g + 1; gg <= g_stop; ++gg) generation(e, ti, gg, work);
+            task_state* s = &e->ts[ti];
+            if (!(s->base <= g && s->end >= g_stop)) continue;
+            float* work = (float*)malloc(sizeof(float) * 2 * (size_t)s->t.dim);
+            for (int gg = g + 1; gg <= g_stop; ++gg) generation(e, ti, MUT_LOCAL_GEN(gg, s->base), work);
             free(work);
         }
         e->gen = g_stop;
@@ -591,6 +661,26 @@ void or_run(or_engine* e, int32_t n_gens, int32_t n_threads) {
 }
 
 int32_t or_generation(const or_engine* e) { return e->gen; }
+
+/* The donor task `task` draws at the transfer boundary `clock` under the current admissions
+ * (-1 if it receives nothing there). */
+int32_t or_transfer_donor(const or_engine* e, int32_t task, int32_t clock) {
+    const int TI = e->p.transfer_interval;
+    if (TI <= 0 || e->p.transfer_count <= 0 || clock % TI != 0 || task < 0 || task >= e->T) return -1;
+    int* list = (int*)malloc(sizeof(int) * (size_t)e->T);
+    const int nd = donor_list(e, clock, list);
+    int pos = -1;
+    for (int q = 0; q < nd; ++q)
+        if (list[q] == task) pos = q;
+    int dn = -1;
+    if (pos >= 0 && nd >= 2) {
+        const uint32_t w = draw_word(e->p.seed, (uint32_t)task, (uint32_t)MUT_LOCAL_GEN(clock, e->ts[task].base),
+                                     OR_P_MIG, 0u, 0u, 0u);
+        dn = list[donor_pick(w, nd, MUT_DONOR_POS(pos, task))];
+    }
+    free(list);
+    return dn;
+}
 
 /* Single phases on the current state, for the tests that pin them one at a time. */
 void or_phase_exchange(or_engine* e, int32_t task, int32_t gen) {
@@ -603,10 +693,11 @@ void or_phase_migrate(or_engine* e, int32_t task) {
 
 void or_phase_transfer(or_engine* e, int32_t gen) {
     if (e->T < 2 || e->p.transfer_count < 1) return;
+    for (int ti = 0; ti < e->T; ++ti) e->donors[ti] = ti;
     for (int ti = 0; ti < e->T; ++ti) snapshot_elites(e, ti);
     for (int ti = 0; ti < e->T; ++ti) {
         float* work = (float*)malloc(sizeof(float) * 2 * (size_t)e->ts[ti].t.dim);
-        transfer_task(e, ti, gen, work);
+        transfer_task(e, ti, gen, e->donors, e->T, ti, work);
         free(work);
     }
 }

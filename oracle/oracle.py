@@ -105,6 +105,12 @@ def _bind(path):
     L.or_grad32.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, f32p, f32p]
     L.or_create.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, ctypes.POINTER(OrParams)]
     L.or_create.restype = ctypes.c_void_p
+    L.or_create_deferred.argtypes = [ctypes.POINTER(OrTask), ctypes.c_int32, ctypes.POINTER(OrParams)]
+    L.or_create_deferred.restype = ctypes.c_void_p
+    L.or_admit.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.or_admit.restype = ctypes.c_int32
+    L.or_transfer_donor.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+    L.or_transfer_donor.restype = ctypes.c_int32
     L.or_destroy.argtypes = [ctypes.c_void_p]
     L.or_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
     L.or_generation.argtypes = [ctypes.c_void_p]
@@ -226,13 +232,24 @@ def make_params(**kw):
 class Engine:
     """The oracle's multitasking engine over a list of task dicts."""
 
-    def __init__(self, tasks, **params):
+    def __init__(self, tasks, deferred=False, **params):
+        """deferred: online admission, no task runs before admit() starts it."""
         self._holders = [_TaskHolder(t) for t in tasks]
         arr = (OrTask * len(tasks))(*[h.c for h in self._holders])
         self._p = make_params(**params)
         self._L = lib()
-        self._h = self._L.or_create(arr, len(tasks), ctypes.byref(self._p))
+        create = self._L.or_create_deferred if deferred else self._L.or_create
+        self._h = create(arr, len(tasks), ctypes.byref(self._p))
         self.n_tasks = len(tasks)
+
+    def admit(self, first, count, generations):
+        """Start tasks first .. first + count - 1 now for `generations` generations."""
+        if self._L.or_admit(self._h, int(first), int(count), int(generations)) != 0:
+            raise ValueError("bad admission (range, repeated task, or off a transfer boundary)")
+
+    def transfer_donor(self, t, clock):
+        """The donor task t draws at the transfer boundary `clock` (-1: none)."""
+        return self._L.or_transfer_donor(self._h, int(t), int(clock))
 
     def run(self, gens, threads=0):
         self._L.or_run(self._h, int(gens), int(threads))
@@ -330,6 +347,10 @@ class CvrpEngine:
         self._L = lib()
         self._h = self._L.or_cvrp_create(carr, len(arr), ctypes.byref(self._p))
         self.instances = instances
+
+    def transfer_donor(self, t, clock):
+        """The donor task t draws at the transfer boundary `clock` (-1: none)."""
+        return self._L.or_transfer_donor(self._h, int(t), int(clock))
 
     def run(self, gens, threads=0):
         self._L.or_cvrp_run(self._h, int(gens), int(threads))

@@ -18,6 +18,8 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -66,6 +68,7 @@ extern "C" int emtf_launch_elites(const emtf::TaskDev* tasks, int n_tasks, const
                                   int stride, float* out_x, float* out_f, hipStream_t stream);
 extern "C" int emtf_scratch_resolve(int device, float** out, size_t* capacity);
 extern "C" int emtf_launch_gen_counter(int32_t* gen, int32_t value, int32_t add, hipStream_t stream);
+extern "C" int emtf_launch_donors(const emtf::TaskDev* tasks, int n_tasks, int g, int32_t* list, hipStream_t stream);
 
 static const int kKB[] = {2, 4, 7, 8, 12, 13, 16};
 
@@ -240,6 +243,10 @@ struct emtf_handle {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_len = 0;           // generations per replayed epoch
+    // online admission: tasks start at emtf_admit; the generation counter is the clock
+    bool online = false;
+    std::vector<int> ends;       // clock values at which admitted tasks finish (sorted)
+    int32_t* d_donors = nullptr; // [T] donor list of a transfer boundary, then its count
 };
 
 static void free_handle(emtf_handle* h) {
@@ -258,6 +265,7 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_trace_worst);
     (void)hipFree(h->d_elite);
     (void)hipFree(h->d_gen);
+    (void)hipFree(h->d_donors);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
@@ -361,6 +369,16 @@ static int launch(emtf_handle* h, int g0, int g1, int init, hipStream_t stream, 
     a.scratch = h->d_scratch;
     a.Qw = h->d_qw;
     a.p = h->p;
+    if (h->online) {
+        // the donors of this boundary, listed on the device before the epoch's kernels
+        a.donors = h->d_donors;
+        a.n_donors = h->d_donors + h->T;
+        const int TI = h->p.transfer_interval;
+        if (!init && TI > 0 && h->p.transfer_count > 0 && g0 > 0 && g0 % TI == 0) {
+            const int e = emtf_launch_donors(h->d_tasks, h->T, g0, h->d_donors, stream);
+            if (e != 0) return fail(EMTF_E_DEVICE, std::string("donor list: ") + hipGetErrorString((hipError_t)e));
+        }
+    }
     const bool pol = !init && polish_at(h, g1);
     const bool pub = !init && publish_at(h, g1);
     const bool fork = !h->side.empty();
@@ -436,17 +454,151 @@ static int replay_epoch(emtf_handle* h, int g0, int len, hipStream_t s) {
     return EMTF_OK;
 }
 
+// Autotuner decisions kept for later creates of batches with the same launch units on the
+// same kind of device: in memory for the life of the process and, when the environment
+// variable EMTF_TUNE_CACHE names a file, in that file (one line per batch signature,
+// appended), so that another process skips the calibration too.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, std::string> g_tune;  // batch signature -> per unit "use ms0 ms1 ms2"
+bool g_tune_file_read = false;
+
+const char* tune_file() {
+    const char* f = getenv("EMTF_TUNE_CACHE");
+    return f && *f ? f : nullptr;
+}
+}  // namespace
+
+// What the calibration of a batch depends on: the device kind, the knobs that change a
+// generation's work, and per launch unit its variants, task count (rounded up to a power
+// of two), largest dimension and population and objective-cost sum.
+static std::string tune_signature(const emtf_handle* h, const std::vector<emtf::TaskDev>& tasks,
+                                  const std::vector<int>& order) {
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipGetDeviceProperties(&prop, dev);
+    char buf[256];
+    snprintf(buf, sizeof buf, "emtf-tune-1 %s/%d a%d i%d x%d m%d t%d", prop.gcnArchName, prop.multiProcessorCount,
+             h->p.algo, h->p.n_islands, h->p.exchange_interval, h->p.migrate_interval, h->p.transfer_interval);
+    std::string sig = buf;
+    for (const Bucket& b : h->buckets) {
+        int maxd = 0, maxp = 0, n2 = 1;
+        double c = 0.0;
+        for (int i = b.offset; i < b.offset + b.count; ++i) {
+            const emtf::TaskDev& td = tasks[order[i]];
+            maxd = std::max(maxd, td.dim);
+            maxp = std::max(maxp, td.pop);
+            c += func_cost(td.func);
+        }
+        while (n2 < b.count) n2 *= 2;
+        snprintf(buf, sizeof buf, " |%d.%d n%d d%d p%d c%.0f", b.wide ? 1 : 0, b.geo, n2, maxd, maxp, c * 64.0 / b.count);
+        sig += buf;
+        for (int v = 0; v < kVariants; ++v)
+            if (b.v[v].fn) {
+                snprintf(buf, sizeof buf, " v%d:%d.%d.%d", v, b.v[v].wide ? 1 : 0, b.v[v].geo, b.v[v].lds_bytes);
+                sig += buf;
+            }
+    }
+    return sig;
+}
+
+static bool tune_lookup(emtf_handle* h, const std::string& sig) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!g_tune_file_read) {
+        g_tune_file_read = true;
+        if (const char* f = tune_file())
+            if (FILE* fp = fopen(f, "r")) {
+                char line[8192];
+                while (fgets(line, sizeof line, fp)) {
+                    char* tab = strchr(line, '\t');
+                    if (!tab) continue;
+                    *tab = 0;
+                    std::string val = tab + 1;
+                    while (!val.empty() && (val.back() == '\n' || val.back() == '\r')) val.pop_back();
+                    g_tune[line] = val;
+                }
+                fclose(fp);
+            }
+    }
+    auto it = g_tune.find(sig);
+    if (it == g_tune.end()) return false;
+    // parse first, apply only a complete, consistent record
+    std::vector<int> use;
+    std::vector<float> ms;
+    const char* q = it->second.c_str();
+    for (size_t k = 0; k < h->buckets.size(); ++k) {
+        int u = 0, n = 0;
+        float m[kVariants] = {};
+        if (sscanf(q, " %d %f %f %f%n", &u, &m[0], &m[1], &m[2], &n) != 4) return false;
+        if (u < 0 || u >= kVariants || !h->buckets[k].v[u].fn) return false;
+        use.push_back(u);
+        for (int v = 0; v < kVariants; ++v) ms.push_back(m[v]);
+        q += n;
+    }
+    for (size_t k = 0; k < h->buckets.size(); ++k) {
+        h->buckets[k].use = use[k];
+        for (int v = 0; v < kVariants; ++v) h->buckets[k].ms[v] = ms[k * kVariants + v];
+    }
+    return true;
+}
+
+static void tune_store(const emtf_handle* h, const std::string& sig) {
+    std::string val;
+    char buf[96];
+    for (const Bucket& b : h->buckets) {
+        snprintf(buf, sizeof buf, " %d %.6g %.6g %.6g", b.use, b.ms[0], b.ms[1], b.ms[2]);
+        val += buf;
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[sig] = val;
+    if (const char* f = tune_file())
+        if (FILE* fp = fopen(f, "a")) {
+            fprintf(fp, "%s\t%s\n", sig.c_str(), val.c_str());
+            fclose(fp);
+        }
+}
+
+// Units to streams longest-first on the measured times (the caller's stream and up to
+// three more), and the tuner's log line (EMTF_TUNE_LOG=1).
+static void assign_streams(emtf_handle* h, bool cached) {
+    const int n_streams = 1 + (int)h->side.size();
+    std::stable_sort(h->buckets.begin(), h->buckets.end(),
+                     [](const Bucket& x, const Bucket& y) { return x.ms[x.use] > y.ms[y.use]; });
+    std::vector<double> load(n_streams, 0.0);
+    for (Bucket& b : h->buckets) {
+        int s = 0;
+        for (int q = 1; q < n_streams; ++q)
+            if (load[q] < load[s]) s = q;
+        b.stream = s;
+        load[s] += b.ms[b.use];
+    }
+    if (const char* lg = getenv("EMTF_TUNE_LOG"); lg && *lg == '1')
+        for (const Bucket& b : h->buckets) {
+            fprintf(stderr, "emtf tune: %d tasks on stream %d:", b.count, b.stream);
+            for (int v = 0; v < kVariants; ++v)
+                if (b.v[v].fn) fprintf(stderr, " [%d] %s %d %.3f ms", v, b.v[v].name, b.v[v].geo, b.ms[v]);
+            fprintf(stderr, " -> [%d]%s\n", b.use, cached ? " (cached)" : "");
+        }
+}
+
 // Autotuner: with the batch initialised, time every launch unit alone for a few
 // generations with each of its kernel variants (the LDS kernel and the wide kernel
 // compute the same bits), keep the faster, and give the units to the streams
 // longest-first on the measured times.  Populations and fitness are restored after
 // each trial (the trials start at generation 0 and publish nothing), so the run's
-// results do not depend on the calibration.
-static int calibrate(emtf_handle* h, int64_t nX, int64_t nF) {
+// results do not depend on the calibration.  A batch whose signature was calibrated
+// before (this process, or the EMTF_TUNE_CACHE file) takes the recorded decisions.
+static int calibrate(emtf_handle* h, int64_t nX, int64_t nF, const std::vector<int>& order) {
     bool any_alt = false;
     for (const Bucket& b : h->buckets)
         for (int v = 1; v < kVariants; ++v) any_alt = any_alt || b.v[v].fn;
     if (!any_alt && h->buckets.size() < 2) return EMTF_OK;
+    const std::string sig = tune_signature(h, h->tasks, order);
+    if (tune_lookup(h, sig)) {
+        assign_streams(h, true);
+        return EMTF_OK;
+    }
     const int gens = 4;
     float *bx = nullptr, *bf = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -499,24 +651,8 @@ static int calibrate(emtf_handle* h, int64_t nX, int64_t nF) {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (e != hipSuccess) return fail(EMTF_E_DEVICE, std::string("calibration: ") + hipGetErrorString(e));
-    const int n_streams = 1 + (int)h->side.size();
-    std::stable_sort(h->buckets.begin(), h->buckets.end(),
-                     [](const Bucket& x, const Bucket& y) { return x.ms[x.use] > y.ms[y.use]; });
-    std::vector<double> load(n_streams, 0.0);
-    for (Bucket& b : h->buckets) {
-        int s = 0;
-        for (int q = 1; q < n_streams; ++q)
-            if (load[q] < load[s]) s = q;
-        b.stream = s;
-        load[s] += b.ms[b.use];
-    }
-    if (const char* lg = getenv("EMTF_TUNE_LOG"); lg && *lg == '1')
-        for (const Bucket& b : h->buckets) {
-            fprintf(stderr, "emtf tune: %d tasks on stream %d:", b.count, b.stream);
-            for (int v = 0; v < kVariants; ++v)
-                if (b.v[v].fn) fprintf(stderr, " [%d] %s %d %.3f ms", v, b.v[v].name, b.v[v].geo, b.ms[v]);
-            fprintf(stderr, " -> [%d]\n", b.use);
-        }
+    tune_store(h, sig);
+    assign_streams(h, false);
     return EMTF_OK;
 }
 
@@ -549,6 +685,11 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     if (prm->formulas != EMTF_FORMULAS_TEXTBOOK && prm->polish_interval > 0 && prm->polish_steps > 0 &&
         prm->polish_topk > 0)
         return fail(EMTF_E_UNSUPPORTED, "gradient polish needs the textbook formulas");
+    if (prm->admission < EMTF_ADMIT_AT_CREATE || prm->admission > EMTF_ADMIT_ONLINE)
+        return fail(EMTF_E_INVALID, "unknown admission");
+    const bool online = prm->admission == EMTF_ADMIT_ONLINE;
+    if (online && prm->polish_interval > 0 && prm->polish_steps > 0 && prm->polish_topk > 0)
+        return fail(EMTF_E_UNSUPPORTED, "gradient polish is not available with online admission");
 
     int max_dim = 0;
     for (int t = 0; t < n_tasks; ++t) {
@@ -666,7 +807,7 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     // its largest dimension, for a wide bucket, its 512-thread build; [2] the wide kernel
     // with Q in LDS where it fits (timed against [0] by the autotuner; not with polish,
     // whose kernel needs the LDS rows)
-    const bool tune = prm->tune != 1 && !want_polish;
+    const bool tune = prm->tune != 1 && !want_polish && !online;
     std::vector<int> alt_nt(h->buckets.size(), -1);
     for (size_t k = 0; k < h->buckets.size(); ++k) {
         Bucket& b = h->buckets[k];
@@ -742,6 +883,8 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
         td.mb_off = nM;
         td.scr_off = nS;
         td.qw_off = nW;
+        td.gen_base = online ? INT32_MAX : 0;  // not started until admitted
+        td.gen_end = INT32_MAX;
         const int popP = (d.pop + 31) & ~31;
         nX += (int64_t)d.pop * d.dim;
         nF += d.pop;
@@ -915,9 +1058,19 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(order)"); }
     e = hipMalloc((void**)&h->d_gen, sizeof(int32_t));
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(generation counter)"); }
+    h->online = online;
+    if (online) {
+        e = hipMalloc((void**)&h->d_donors, sizeof(int32_t) * ((size_t)n_tasks + 1));
+        if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_NOMEM, "hipMalloc(donor list)"); }
+        // tasks not admitted yet read back as zero rows with NaN fitness
+        e = hipMemset(h->d_X, 0, sizeof(float) * (size_t)std::max<int64_t>(nX, 1));
+        if (e == hipSuccess) e = hipMemset(h->d_fit, 0xFF, sizeof(float) * (size_t)std::max<int64_t>(nF, 1));
+        if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, "initial state of an online handle"); }
+    }
     // replayed epochs unless the caller asks for direct launches; the polish kernel's epochs
-    // are launched directly
-    h->use_graph = prm->schedule == EMTF_SCHEDULE_AUTO && !want_polish;
+    // and those of online admission (whose launches change with the running tasks) are
+    // launched directly
+    h->use_graph = prm->schedule == EMTF_SCHEDULE_AUTO && !want_polish && !online;
 #define CP(dst, src, bytes)                                                                    \
     do {                                                                                       \
         hipError_t e2 = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);                     \
@@ -930,10 +1083,10 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     CP(h->d_tab, TB.data(), sizeof(float) * TB.size());
     if (nW > 0) CP(h->d_qw, QW.data(), sizeof(float) * QW.size());
 #undef CP
-    if ((rc = launch(h, 0, 0, 1, nullptr)) != EMTF_OK) { free_handle(h); return rc; }
+    if (!online && (rc = launch(h, 0, 0, 1, nullptr)) != EMTF_OK) { free_handle(h); return rc; }
     e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) { free_handle(h); return fail(EMTF_E_DEVICE, std::string("init: ") + hipGetErrorString(e)); }
-    if (tune && prm->tune == 0 && (rc = calibrate(h, nX, nF)) != EMTF_OK) { free_handle(h); return rc; }
+    if (tune && prm->tune == 0 && (rc = calibrate(h, nX, nF, order)) != EMTF_OK) { free_handle(h); return rc; }
     if (tune && prm->tune >= 2)
         for (Bucket& b : h->buckets) {
             const int v = prm->tune - 1;
@@ -963,6 +1116,9 @@ int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
             const int nb = (h->gen / PI + 1) * PI;
             if (nb < g1) g1 = nb;
         }
+        // online: an epoch also ends where an admitted task finishes
+        const auto ne = std::upper_bound(h->ends.begin(), h->ends.end(), h->gen);
+        if (ne != h->ends.end() && *ne < g1) g1 = *ne;
         const int TR = h->trace_points < h->trace_max ? h->trace_interval : 0;
         if (TR > 0) {
             const int nb = h->trace_g0 + ((h->gen - h->trace_g0) / TR + 1) * TR;
@@ -990,6 +1146,29 @@ int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
         }
     }
     return EMTF_OK;
+}
+
+int emtf_admit(emtf_handle* h, int32_t first, int32_t count, int32_t generations, void* stream) {
+    g_err.clear();
+    if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
+    if (!h->online) return fail(EMTF_E_INVALID, "the handle was created with admission AT_CREATE");
+    if (first < 0 || count < 1 || first > h->T - count) return fail(EMTF_E_INVALID, "task range out of bounds");
+    if (generations < 0 || generations > INT32_MAX - 1 - h->gen) return fail(EMTF_E_INVALID, "bad generation count");
+    const int TI = h->p.transfer_interval;
+    if (TI > 0 && h->gen % TI != 0) return fail(EMTF_E_INVALID, "admission off a transfer boundary");
+    for (int t = first; t < first + count; ++t)
+        if (h->tasks[t].gen_base != INT32_MAX) return fail(EMTF_E_INVALID, "task admitted before");
+    hipStream_t s = (hipStream_t)stream;
+    if (const int rc = use_stream(h, s); rc != EMTF_OK) return rc;
+    for (int t = first; t < first + count; ++t) {
+        h->tasks[t].gen_base = h->gen;
+        h->tasks[t].gen_end = h->gen + generations;
+    }
+    h->ends.insert(std::upper_bound(h->ends.begin(), h->ends.end(), h->gen + generations), h->gen + generations);
+    // the descriptors' host copies stay as written for the life of the handle
+    HIP_OR_FAIL(hipMemcpyAsync(h->d_tasks + first, h->tasks.data() + first, sizeof(emtf::TaskDev) * count,
+                               hipMemcpyHostToDevice, s));
+    return launch(h, h->gen, h->gen, 1, s);
 }
 
 int emtf_trace_start(emtf_handle* h, int32_t interval, int32_t max_points, void* stream) {

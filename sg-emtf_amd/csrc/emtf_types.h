@@ -20,7 +20,11 @@ enum Algo : int32_t { A_DE_RAND_1_BIN = 0, A_DE_CURRENT_1_BIN = 1, A_GA_SBX_PM =
 struct TaskDev {
     int32_t func, dim, pop, xform;
     float lo, hi;
-    int32_t rsv0, rsv1;
+    // life on the handle's generation clock: the task produces generations gen_base + 1 ..
+    // gen_end (0 and INT32_MAX unless admitted online; a task not yet admitted has
+    // gen_base = INT32_MAX); its own generation counter, which addresses its draws, is
+    // the clock minus gen_base
+    int32_t gen_base, gen_end;
     int64_t x_off;    // population X[pop][dim], row-major
     int64_t fit_off;  // fitness[pop]
     int64_t q_off;    // transform matrix Q[dim][dim], row-major (identity / diagonal materialised)
@@ -62,6 +66,11 @@ struct EpochArgs {
     const int32_t* gen_dev;
     int32_t polish_waves;  // polish kernel: waves working on elites (1..4)
     int32_t polish_cpe;    // polish kernel: MFMA columns per elite (power of two, 1..32)
+    // Transfer donors (online admission): donors[0 .. *n_donors) are the ids, ascending, of
+    // the tasks that ran up to the transfer boundary g0 and run on after it; null: every
+    // task of the handle
+    const int32_t* donors;
+    const int32_t* n_donors;
 };
 
 // Stand-alone objective / adjoint evaluation of n points (device pointers).

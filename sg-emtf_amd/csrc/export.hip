@@ -102,6 +102,41 @@ __global__ __launch_bounds__(256) void trace_kernel(const TaskDev* __restrict__ 
     }
 }
 
+// The transfer donors of online admission at the boundary g: the ids, ascending, of the
+// tasks that ran up to g and run on after it (gen_base < g < gen_end), then their count
+// at list[n_tasks].  One workgroup of 1024 threads, 1024 tasks a round: each wave counts
+// its members with a ballot, the waves' counts give each member its slot.
+__global__ __launch_bounds__(1024) void donor_kernel(const TaskDev* __restrict__ tasks, int n_tasks, int g,
+                                                     int32_t* __restrict__ list) {
+    __shared__ int wsum[16];
+    __shared__ int done;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) done = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < n_tasks; c0 += 1024) {
+        const int t = c0 + tid;
+        bool in = false;
+        if (t < n_tasks) {
+            const TaskDev td = tasks[t];
+            in = td.gen_base < g && g < td.gen_end;
+        }
+        const unsigned long long bal = __ballot(in);
+        if (lane == 0) wsum[w] = __popcll(bal);
+        __syncthreads();
+        int off = done;
+        for (int q = 0; q < w; ++q) off += wsum[q];
+        if (in) list[off + __popcll(bal & ((1ull << lane) - 1ull))] = t;
+        __syncthreads();
+        if (tid == 0) {
+            int tot = 0;
+            for (int q = 0; q < 16; ++q) tot += wsum[q];
+            done += tot;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) list[n_tasks] = done;
+}
+
 // The device generation counter of replayed epochs: set (after directly launched
 // epochs) or advanced by one epoch (the last node of a captured epoch).
 __global__ void gen_counter_kernel(int32_t* gen, int32_t value, int32_t add) {
@@ -112,6 +147,12 @@ __global__ void gen_counter_kernel(int32_t* gen, int32_t value, int32_t add) {
 
 extern "C" int emtf_launch_gen_counter(int32_t* gen, int32_t value, int32_t add, hipStream_t stream) {
     hipLaunchKernelGGL(emtf::gen_counter_kernel, dim3(1), dim3(64), 0, stream, gen, value, add);
+    return (int)hipGetLastError();
+}
+
+extern "C" int emtf_launch_donors(const emtf::TaskDev* tasks, int n_tasks, int g, int32_t* list,
+                                  hipStream_t stream) {
+    hipLaunchKernelGGL(emtf::donor_kernel, dim3(1), dim3(1024), 0, stream, tasks, n_tasks, g, list);
     return (int)hipGetLastError();
 }
 

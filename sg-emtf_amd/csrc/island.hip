@@ -92,6 +92,39 @@ __device__ __forceinline__ void rebase_epoch(EpochArgs& a) {
     }
 }
 
+// Online admission: a task takes part in the epochs that lie inside its life and is
+// initialised by the launch at its admission (every task of a handle created without
+// online admission: gen_base 0, gen_end INT32_MAX).
+__device__ __forceinline__ bool runs_epoch(const EpochArgs& a, const TaskDev& td) {
+    return a.init ? td.gen_base == a.g0 : (td.gen_base <= a.g0 && a.g1 <= td.gen_end);
+}
+
+// Whether the task receives elites at the start of this epoch: a transfer boundary that
+// is not its own start, with another donor to draw.
+__device__ __forceinline__ bool transfer_now(const EpochArgs& a, const TaskDev& td) {
+    const int TI = a.p.transfer_interval;
+    if (a.init || TI <= 0 || a.p.transfer_count <= 0 || a.g0 % TI != 0 || a.g0 <= td.gen_base) return false;
+    return a.donors ? *a.n_donors > 1 : a.n_tasks > 1;
+}
+
+// The donor of task `task` at a transfer event: the k-th of the other donors in ascending
+// id order, k = w mod (n - 1) -- every task of the handle, or the online donor list.
+__device__ __forceinline__ int pick_donor(const EpochArgs& a, int task, uint32_t w) {
+    if (!a.donors) {
+        const int dn = (int)(w % (uint32_t)(a.n_tasks - 1));
+        return dn >= task ? dn + 1 : dn;
+    }
+    const int n = *a.n_donors;
+    const int k = (int)(w % (uint32_t)(n - 1));
+    int lo = 0, hi = n - 1;  // the task's own position in the list (it is a donor itself)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a.donors[mid] < task) lo = mid + 1;
+        else hi = mid;
+    }
+    return a.donors[k + (k >= lo ? 1 : 0)];
+}
+
 // ---------------------------------------------------------------------------
 // Objective of the lane's individual from its MFMA accumulators.
 // Lane (col, h) holds z_j for j = 32 jt + 8 (r >> 2) + 4 h + (r & 3); the two
@@ -494,9 +527,10 @@ __device__ __forceinline__ GaConst ga_const(const EpochArgs& a, const TaskDev& t
 }
 
 // The lane's trial half-row, block by block, each block's four genes fed straight into
-// the MFMA k-steps of z = Q (x - o): the next block's draws, loads and arithmetic are
-// independent of the matrix core's work and can issue while it runs.  The same genes,
-// k order and accumulators as generating the whole row first and then evaluating it.
+// the MFMA k-steps of z = Q (x - o), so no lane holds a whole row (on gfx950 the FP32
+// MFMA and the VALU share the SIMD's issue: measured, neither overlaps the other, so the
+// interleaving saves registers, not time).  The same genes, k order and accumulators as
+// generating the whole row first and then evaluating it.
 // The trial genes are left in tr (for the selection copy); returns f of the trial.
 template <int KB, int FUNC, int ALGO>
 __device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs& a, const TaskDev& td, const Geo& g,
@@ -732,15 +766,15 @@ template <int KB, int FUNC>
 __device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
                                             const Rng& rng, int tid) {
     using S = Shape<KB>;
-    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0;
+    // draws at the task's own generation g0, the mailbox buffer by the handle's clock
+    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0 - td.gen_base;
     const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
     rank_islands(L, g.pop, g.pop, tid);
     __syncthreads();
     const uint32_t wd = rng.word((uint32_t)g0, P_MIG, 0u, 0u, 0u);
-    int dn = (int)(wd % (uint32_t)(a.n_tasks - 1));
-    if (dn >= (int)rng.task) dn += 1;
+    const int dn = pick_donor(a, (int)rng.task, wd);
     const TaskDev dd = a.tasks[dn];
-    const float* mb = a.mailbox + (int64_t)((g0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
+    const float* mb = a.mailbox + (int64_t)((a.g0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
     const bool same_box = (dd.lo == td.lo && dd.hi == td.hi);
     const float rng_s = td.hi - td.lo, rng_d = dd.hi - dd.lo;
     const int ntc = (k + 31) / 32;
@@ -1239,11 +1273,10 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
     __syncthreads();
 
     const int TI = a.p.transfer_interval;
-    const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
-    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) transfer_in<KB, FUNC>(a, td, g, L, rng, tid);
+    if (transfer_now(a, td)) transfer_in<KB, FUNC>(a, td, g, L, rng, tid);
 
     const bool islands = a.p.n_islands > 1;
-    for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
+    for (int gen = a.g0 - td.gen_base + 1; gen <= a.g1 - td.gen_base; ++gen) {
         if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
             island_exchange<KB>(a, g, L, a.scratch + td.scr_off, rng, gen, tid);
         generation_step<KB, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
@@ -1290,6 +1323,7 @@ __global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(E
     rebase_epoch(a);
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
+    if (!runs_epoch(a, td)) return;
     switch (td.func) {
     case F_SPHERE: epoch_body<KB, F_SPHERE, ALGO>(a, smem, task, td); break;
     case F_ROSENBROCK: epoch_body<KB, F_ROSENBROCK, ALGO>(a, smem, task, td); break;
@@ -1590,15 +1624,14 @@ __device__ __forceinline__ void wide_transfer_in(const EpochArgs& a, const TaskD
                                                  float* T, const Rng& rng, int tid, int nsb) {
     using S = WShape<NT>;
     constexpr int RS = S::RS, HS = S::HS;
-    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0;
+    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0 - td.gen_base;
     const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
     rank_islands<WT>(L, g.pop, g.pop, tid);
     __syncthreads();
     const uint32_t wd = rng.word((uint32_t)g0, P_MIG, 0u, 0u, 0u);
-    int dn = (int)(wd % (uint32_t)(a.n_tasks - 1));
-    if (dn >= (int)rng.task) dn += 1;
+    const int dn = pick_donor(a, (int)rng.task, wd);
     const TaskDev dd = a.tasks[dn];
-    const float* mb = a.mailbox + (int64_t)((g0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
+    const float* mb = a.mailbox + (int64_t)((a.g0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
     const bool same_box = (dd.lo == td.lo && dd.hi == td.hi);
     const float rng_s = td.hi - td.lo, rng_d = dd.hi - dd.lo;
     const float* qrow = L.Q + col * RS + h * HS;
@@ -1729,11 +1762,10 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     }
 
     const int TI = a.p.transfer_interval;
-    const bool xfer_on = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
-    if (!a.init && xfer_on && a.g0 > 0 && a.g0 % TI == 0) wide_transfer_in<NT, FUNC, WT>(a, td, g, L, T, rng, tid, nsb);
+    if (transfer_now(a, td)) wide_transfer_in<NT, FUNC, WT>(a, td, g, L, T, rng, tid, nsb);
 
     const bool islands = a.p.n_islands > 1;
-    for (int gen = a.g0 + 1; gen <= a.g1; ++gen) {
+    for (int gen = a.g0 - td.gen_base + 1; gen <= a.g1 - td.gen_base; ++gen) {
         if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
             island_exchange<4 * NT, WT>(a, g, L, T, rng, gen, tid);
         wide_generation<NT, FUNC, ALGO, WT, QL>(a, td, g, L, T, rng, gen, tid, nsb);
@@ -1774,6 +1806,7 @@ __global__ __launch_bounds__(WT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     rebase_epoch(a);
     const int task = order[blockIdx.x];
     const TaskDev td = a.tasks[task];
+    if (!runs_epoch(a, td)) return;
     switch (td.func) {
     case F_SPHERE: wide_body<NT, F_SPHERE, ALGO, WT, QL>(a, smem, task, td); break;
     case F_ROSENBROCK: wide_body<NT, F_ROSENBROCK, ALGO, WT, QL>(a, smem, task, td); break;

@@ -29,7 +29,7 @@ class Params(ctypes.Structure):
                 ("transfer_interval", ctypes.c_int32), ("transfer_count", ctypes.c_int32),
                 ("transfer_cr", ctypes.c_float), ("seed", ctypes.c_uint64), ("polish_interval", ctypes.c_int32),
                 ("polish_steps", ctypes.c_int32), ("polish_topk", ctypes.c_int32), ("schedule", ctypes.c_int32), ("tune", ctypes.c_int32),
-                ("formulas", ctypes.c_int32)]
+                ("formulas", ctypes.c_int32), ("admission", ctypes.c_int32)]
 
 
 class EmtfError(RuntimeError):
@@ -51,6 +51,7 @@ def lib():
         vp = ctypes.c_void_p
         L.emtf_create.argtypes = [ctypes.POINTER(TaskDesc), ctypes.c_int32, ctypes.POINTER(Params), ctypes.POINTER(vp)]
         L.emtf_run.argtypes = [vp, ctypes.c_int32, vp]
+        L.emtf_admit.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp]
         L.emtf_sync.argtypes = [vp]
         L.emtf_generation.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.emtf_get_population.argtypes = [vp, ctypes.c_int32, f32p, f32p]
@@ -61,7 +62,7 @@ def lib():
         L.emtf_trace.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), f32p, f32p]
         L.emtf_destroy.argtypes = [vp]
         L.emtf_last_error.restype = ctypes.c_char_p
-        for n in ["emtf_create", "emtf_run", "emtf_sync", "emtf_generation", "emtf_get_population",
+        for n in ["emtf_create", "emtf_run", "emtf_admit", "emtf_sync", "emtf_generation", "emtf_get_population",
                   "emtf_set_population", "emtf_get_best", "emtf_get_elites", "emtf_trace_start", "emtf_trace",
                   "emtf_destroy"]:
             getattr(L, n).restype = ctypes.c_int

@@ -20,11 +20,13 @@ SCHEDULE_AUTO, SCHEDULE_EPOCHS, SCHEDULE_PERSISTENT = 0, 1, 2
 # objective formulas: this engine's binary32 forms, or the reference's own (its DE file's
 # evaluation(), or the GA file's copy whose Griewank has no product)
 FORMULAS_TEXTBOOK, FORMULAS_REFERENCE_DE, FORMULAS_REFERENCE_GA = 0, 1, 2
+# when tasks start: at create, or online (each batch at its emtf_admit)
+ADMIT_AT_CREATE, ADMIT_ONLINE = 0, 1
 
 DEFAULT_PARAMS = dict(algo=DE_RAND_1_BIN, F=0.5, CR=0.9, eta_c=2.0, eta_m=5.0, pm=1.0, n_islands=1,
                       exchange_interval=0, migrate_interval=0, migrate_count=0, transfer_interval=0,
                       transfer_count=0, transfer_cr=0.0, seed=0, polish_interval=0, polish_steps=0, polish_topk=0,
-                      schedule=0, tune=0, formulas=0)
+                      schedule=0, tune=0, formulas=0, admission=0)
 
 
 def _fp(a):
@@ -64,6 +66,12 @@ class Engine:
     def run(self, generations, stream=None):
         """Queue `generations` generations (asynchronous on `stream`, an int handle or None)."""
         _lib.check(_lib.lib().emtf_run(self._h, int(generations), ctypes.c_void_p(stream or 0)))
+
+    def admit(self, first, count, generations, stream=None):
+        """Online admission: start tasks first .. first + count - 1 at the current clock for
+        `generations` generations (asynchronous on `stream`)."""
+        _lib.check(_lib.lib().emtf_admit(self._h, int(first), int(count), int(generations),
+                                         ctypes.c_void_p(stream or 0)))
 
     def sync(self):
         _lib.check(_lib.lib().emtf_sync(self._h))

@@ -4,27 +4,24 @@ The reference's driver (proposed model/SG-EMTF.cu:1842-1958) splits its T tasks
 into BATCH_NUM batches at random cut points, lets batch i arrive after a random
 gap of U[0, 100) ms, starts one host thread per task as its batch arrives and
 reports the mean response time (arrival -> start) and solve time (arrival ->
-finished) per task.
+finished) per task; a task's transfer partner is drawn among every task still
+running (:1682-1699).
 
-Here a batch is one engine handle created when the batch arrives and run on its
-own non-blocking stream, so batches overlap on the device with the ones already
-running; nothing waits on the host except for the next arrival.  Tasks of a batch
-exchange elites among themselves (a handle's transfer partners are its own
-tasks).  Batch b runs with seed `batch_seed(seed, b)`, so its draws are those of a
-plain Engine over the same tasks and seed: the results are bit-exact with the
-oracle per batch.
+Here all tasks live in one engine handle created with online admission: the
+handle's generation counter is a clock that advances in chunks queued on one
+stream, and a batch that has arrived is admitted (emtf_admit) at the clock the
+queued work has reached, so it waits behind at most `depth` chunks.  Its tasks
+then run their own generations 0.. and exchange elites with every task running
+at a transfer boundary, whichever batch it came with.  The clock at which each
+batch was admitted is recorded (`admitted_at`): replaying the same admissions on
+the oracle reproduces every population bit for bit.
 """
 import time
 
 import numpy as np
 
 from . import hip
-from .engine import Engine
-
-
-def batch_seed(seed, b):
-    """The Philox seed of batch b (distinct streams for tasks that share an index)."""
-    return (int(seed) + int(b) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+from .engine import ADMIT_ONLINE, Engine
 
 
 def arrival_schedule(n_tasks, n_batches, max_gap_ms=100.0, seed=0):
@@ -45,12 +42,10 @@ def arrival_schedule(n_tasks, n_batches, max_gap_ms=100.0, seed=0):
 class OnlineRun:
     """Admit batches of tasks at their arrival times and run each for `generations`.
 
-    Each running batch keeps at most `depth` chunks of `chunk` generations queued on its
-    stream; the host tops them up as they finish.  Long backlogs would otherwise sit in the
-    few hardware queues that all streams share, and a new batch's first kernels would wait
-    behind them."""
+    The clock advances in chunks of `chunk` generations (a multiple of the transfer
+    interval, where a batch may be admitted), at most `depth` of them queued ahead."""
 
-    def __init__(self, tasks, cuts, arrival_ms, generations, seed=0, chunk=100, depth=2, **params):
+    def __init__(self, tasks, cuts, arrival_ms, generations, seed=0, chunk=20, depth=2, **params):
         self.tasks = tasks
         self.cuts = np.asarray(cuts, int)
         self.arrival_ms = np.asarray(arrival_ms, float)
@@ -60,59 +55,53 @@ class OnlineRun:
             raise ValueError("arrivals must be non-decreasing")
         if chunk < 1 or depth < 1:
             raise ValueError("chunk and depth must be >= 1")
+        ti = int(params.get("transfer_interval", 0))
+        if ti > 0 and chunk % ti:
+            raise ValueError("chunk must be a multiple of the transfer interval")
         self.generations = int(generations)
-        self.seed = seed
         self.chunk, self.depth = int(chunk), int(depth)
-        # the autotuner's create-time trials free device buffers (a device-wide
-        # synchronisation that would wait for the batches already running) and would be
-        # timed against them: off unless asked for
-        self.params = dict(params)
-        self.params.setdefault("tune", 1)
-        self.engines = []
-        self.streams = []   # a handle keeps using its run's stream: they live as long as it does
+        self.params = dict(params, seed=seed)
+        self.engine = Engine(tasks, admission=ADMIT_ONLINE, **self.params)
+        self.stream = hip.Stream()
+        self.admitted_at = np.full(len(self.arrival_ms), -1, int)  # clock of each admission
 
     def run(self):
-        """Returns per-task response_ms (arrival -> first generations queued, creation
-        included) and solve_ms (arrival -> last generation finished)."""
+        """Returns per-task response_ms (arrival -> admitted, its initial population queued)
+        and solve_ms (arrival -> its last generation finished)."""
         n_b = len(self.arrival_ms)
         resp = np.zeros(n_b)
         solve = np.full(n_b, np.nan)
-        queued = [0] * n_b
-        pending = [[] for _ in range(n_b)]
+        eng, st = self.engine, self.stream.handle
+        clock = 0          # the clock the queued work reaches
+        queued = []        # (event, clock at its end)
+        nxt = 0            # next batch to arrive
         t0 = time.perf_counter()
 
         def now_ms():
             return (time.perf_counter() - t0) * 1e3
 
-        def pump():
-            for b, eng in enumerate(self.engines):
-                if not np.isnan(solve[b]):
-                    continue
-                while pending[b] and pending[b][0].query():
-                    pending[b].pop(0)
-                if queued[b] == self.generations and not pending[b]:
-                    solve[b] = now_ms() - self.arrival_ms[b]
-                    continue
-                while len(pending[b]) < self.depth and queued[b] < self.generations:
-                    g = min(self.chunk, self.generations - queued[b])
-                    eng.run(g, stream=self.streams[b].handle)
-                    ev = hip.Event()
-                    ev.record(self.streams[b].handle)
-                    pending[b].append(ev)
-                    queued[b] += g
-
-        for b in range(n_b):
-            while now_ms() < self.arrival_ms[b]:
-                pump()
-                time.sleep(5e-5)
-            lo, hi = self.cuts[b], self.cuts[b + 1]
-            self.engines.append(Engine(self.tasks[lo:hi], seed=batch_seed(self.seed, b), **self.params))
-            self.streams.append(hip.Stream())
-            pump()
-            resp[b] = now_ms() - self.arrival_ms[b]
         while np.isnan(solve).any():
-            pump()
-            time.sleep(5e-5)
+            while nxt < n_b and now_ms() >= self.arrival_ms[nxt]:
+                lo, hi = self.cuts[nxt], self.cuts[nxt + 1]
+                eng.admit(lo, hi - lo, self.generations, stream=st)
+                self.admitted_at[nxt] = clock
+                resp[nxt] = now_ms() - self.arrival_ms[nxt]
+                nxt += 1
+            while queued and queued[0][0].query():
+                done = queued.pop(0)[1]
+                for b in range(nxt):
+                    if np.isnan(solve[b]) and self.admitted_at[b] + self.generations <= done:
+                        solve[b] = now_ms() - self.arrival_ms[b]
+            running = any(self.admitted_at[b] + self.generations > clock for b in range(nxt))
+            if running and len(queued) < self.depth:
+                eng.run(self.chunk, stream=st)
+                ev = hip.Event()
+                ev.record(st)
+                clock += self.chunk
+                queued.append((ev, clock))
+            else:
+                time.sleep(5e-5)
+        self.clock = clock
         self.makespan_ms = now_ms()
         per_task = np.repeat(np.arange(n_b), np.diff(self.cuts))
         self.response_ms = resp[per_task]
@@ -121,9 +110,7 @@ class OnlineRun:
 
     def best(self):
         """Best fitness of every task, in the original task order."""
-        return np.concatenate([e.best()[0] for e in self.engines])
+        return self.engine.best()[0]
 
     def close(self):
-        for e in self.engines:
-            e.close()
-        self.engines, self.streams = [], []
+        self.engine.close()

@@ -99,6 +99,38 @@ def test_invalid_params():
     assert "even island" in _create([d2], algo=2, n_islands=2)[1]
 
 
+def test_online_admission_arguments():
+    import emtf
+    from emtf import tasks as tg
+    d, keep = _desc()
+    assert "admission" in _create([d], admission=3)[1]
+    assert "online" in _create([d, d], admission=1, polish_interval=5, polish_steps=2, polish_topk=1)[1]
+    from emtf import _lib
+    L = _lib.lib()
+    assert L.emtf_admit(None, 0, 1, 10, None) == -1
+    del emtf, tg
+
+
+def test_params_layout_matches_the_header(tmp_path):
+    """The ctypes mirror of emtf_params has the C struct's size and field offsets."""
+    import shutil
+    import subprocess
+    from emtf import _lib
+    gcc = shutil.which("gcc")
+    if not gcc:
+        pytest.skip("gcc missing")
+    names = [f[0] for f in _lib.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"emtf.h\"\nint main(void) {\n"
+                   + "".join(f'    printf("%zu\\n", offsetof(emtf_params, {n}));\n' for n in names)
+                   + '    printf("%zu\\n", sizeof(emtf_params));\n    return 0;\n}\n')
+    exe = str(tmp_path / "layout")
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", exe], check=True)
+    got = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    want = [getattr(_lib.Params, n).offset for n in names] + [ctypes.sizeof(_lib.Params)]
+    assert got == want
+
+
 def test_null_arguments():
     from emtf import _lib
     L = _lib.lib()

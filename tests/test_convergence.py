@@ -160,23 +160,3 @@ def test_arrival_schedule():
     assert online.arrival_schedule(5, 1)[0].tolist() == [0, 5]
     with pytest.raises(ValueError):
         online.arrival_schedule(3, 4)
-
-
-@pytest.mark.gpu
-def test_online_batches_are_bit_exact(oracle_mod):
-    """Batches admitted while earlier ones run: each batch's populations equal the oracle's
-    for that batch's tasks and seed; response <= solve for every task."""
-    from emtf import online
-    tasks = tg.uneven_tasks(24, seed=6, dmax=120, pmax=128)
-    cuts, arr = online.arrival_schedule(len(tasks), 3, max_gap_ms=20.0, seed=1)
-    params = dict(transfer_interval=5, transfer_count=2)
-    run = online.OnlineRun(tasks, cuts, arr, 40, seed=11, **params)
-    resp, solve = run.run()
-    assert np.all(resp >= 0) and np.all(solve >= resp)
-    for b in range(3):
-        lo, hi = cuts[b], cuts[b + 1]
-        orc = oracle_mod.Engine(tasks[lo:hi], seed=online.batch_seed(11, b), **params)
-        orc.run(40)
-        for t in range(hi - lo):
-            assert np.array_equal(run.engines[b].population(t)[0].view(np.uint32), orc.population(t)[0].view(np.uint32))
-    run.close()

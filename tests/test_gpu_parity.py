@@ -186,6 +186,70 @@ def test_autotune_is_bit_exact(oracle_mod, tune):
         _compare(gpu, orc, f"tune={tune} gen {orc.generation}")
 
 
+_TUNE_CHILD = r"""
+import os, sys
+sys.path.insert(0, os.environ["EMTF_PKG"])
+import numpy as np
+import emtf
+from emtf import tasks as tg
+tasks = tg.uneven_tasks(24, seed=31, dmax=180, pmax=192)
+eng = emtf.Engine(tasks, seed=4, tune=0, transfer_interval=3, transfer_count=2)
+eng.run(6)
+f = np.concatenate([eng.population(t)[1] for t in range(len(tasks))])
+np.save(sys.argv[1], f)
+"""
+
+
+def test_tuner_decisions_are_cached(oracle_mod, tmp_path, monkeypatch, capfd):
+    """The autotuner's decisions for a batch are reused: a second create in the same process
+    and a create in a fresh process reading the EMTF_TUNE_CACHE file skip the calibration
+    (the tuner's log says "cached"), pick the same variants and compute the oracle's bits."""
+    import os
+    import subprocess
+    import sys
+    cache = tmp_path / "tune.txt"
+    env = dict(os.environ, EMTF_TUNE_CACHE=str(cache), EMTF_TUNE_LOG="1",
+               EMTF_PKG=os.path.dirname(os.path.dirname(emtf.__file__)))
+    runs = []
+    for k in range(2):
+        print(f"case: tuner cache, fresh process {k}", flush=True)
+        out = tmp_path / f"f{k}.npy"
+        r = subprocess.run([sys.executable, "-c", _TUNE_CHILD, str(out)], env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        runs.append((r.stderr, np.load(out)))
+    lines0 = [l for l in runs[0][0].splitlines() if l.startswith("emtf tune:")]
+    lines1 = [l for l in runs[1][0].splitlines() if l.startswith("emtf tune:")]
+    assert lines0 and not any("(cached)" in l for l in lines0)
+    assert lines1 and all(l.endswith("(cached)") for l in lines1)
+    # the same decisions: identical lines but for the marker
+    assert [l.replace(" (cached)", "") for l in lines1] == lines0
+    assert len(cache.read_text().splitlines()) == 1
+    tasks = tg.uneven_tasks(24, seed=31, dmax=180, pmax=192)
+    orc = oracle_mod.Engine(tasks, seed=4, transfer_interval=3, transfer_count=2)
+    orc.run(6)
+    fo = np.concatenate([orc.population(t)[1] for t in range(len(tasks))])
+    for _, f in runs:
+        assert np.array_equal(f.view(np.uint32), fo.view(np.uint32))
+    # in this process: the second create of a batch takes the first one's decisions
+    monkeypatch.setenv("EMTF_TUNE_CACHE", "")
+    monkeypatch.setenv("EMTF_TUNE_LOG", "1")
+    tasks = tg.uneven_tasks(24, seed=32, dmax=180, pmax=192)
+    orc = oracle_mod.Engine(tasks, seed=4, transfer_interval=3, transfer_count=2)
+    orc.run(6)
+    capfd.readouterr()
+    a = emtf.Engine(tasks, seed=4, tune=0, transfer_interval=3, transfer_count=2)
+    la = [l for l in capfd.readouterr().err.splitlines() if l.startswith("emtf tune:")]
+    b = emtf.Engine(tasks, seed=4, tune=0, transfer_interval=3, transfer_count=2)
+    lb = [l for l in capfd.readouterr().err.splitlines() if l.startswith("emtf tune:")]
+    assert la and not any("(cached)" in l for l in la)
+    assert lb and all(l.endswith("(cached)") for l in lb)
+    a.run(6)
+    b.run(6)
+    _compare(a, orc, "tuned")
+    _compare(b, orc, "tuned from the in-process cache")
+
+
 @pytest.mark.parametrize("schedule", [emtf.SCHEDULE_AUTO, emtf.SCHEDULE_EPOCHS])
 def test_graph_replay_bit_exact(oracle_mod, schedule):
     """Epochs replayed from the captured hipGraph (AUTO) and launched directly (EPOCHS) both

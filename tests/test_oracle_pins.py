@@ -13,7 +13,9 @@ Three kinds of check, none of which reads the oracle's own code:
     the box corners map onto the box corners exactly, SG-EMTF.cu:1555) and replaces own
     rank pop-k+j (SG-EMTF.cu:1568-1589);
   * planted mistakes: the oracle built with -DOR_MUTANT=k (a wrong rank, a non-strict
-    comparison, an offset per island, swapped box ranges, a wrong target) fails the
+    comparison, an offset per island, swapped box ranges, a wrong target; for online
+    admission a finished or just-admitted task taken as donor, the donor list skipping
+    the task's id instead of its position, draws addressed by the clock) fails the
     property check meant to catch it.
 """
 import json
@@ -22,6 +24,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
+
+import test_online
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -279,6 +283,22 @@ def test_exchange_keeps_the_best_over_a_run(oracle_mod):
             best[t] = b
 
 
+def check_online_finished(o):
+    test_online.test_finished_tasks_are_not_donors(o)
+
+
+def check_online_new(o):
+    test_online.test_new_and_pending_tasks_are_not_donors_transfer_crosses_batches(o)
+
+
+def check_online_pick(o):
+    test_online.test_donor_is_the_kth_other_donor(o)
+
+
+def check_online_late(o):
+    test_online.test_late_task_runs_its_own_generations(o)
+
+
 # ------------------------------------------------------------- planted mistakes
 MUTANTS = {
     1: ("exchange offset drawn per (island, slot)", check_exchange),
@@ -288,6 +308,10 @@ MUTANTS = {
     5: ("transfer box map with the ranges swapped", check_transfer),
     6: ("transfer replaces own rank j", check_transfer),
     7: ("CVRP migration writes rank m-1-j", check_cvrp_migration),
+    8: ("online: a task done at the boundary is still a donor", check_online_finished),
+    9: ("online: a task admitted at the boundary is already a donor", check_online_new),
+    10: ("online: the donor skips the task's id, not its list position", check_online_pick),
+    11: ("online: draws addressed by the clock, not the task's generation", check_online_late),
 }
 
 
