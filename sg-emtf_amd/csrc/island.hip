@@ -130,9 +130,22 @@ __device__ __forceinline__ int pick_donor(const EpochArgs& a, int task, uint32_t
 // Lane (col, h) holds z_j for j = 32 jt + 8 (r >> 2) + 4 h + (r & 3); the two
 // halves accumulate the even / odd blocks of four rows in increasing j and
 // are combined once.
+// ZPAD: the caller's Q rows past D are zero, so z_j = +0 there; the objectives whose
+// term at z = +0 is +0 (whose sums it leaves as they are, bit for bit: they start at
+// +0 and never reach -0) and costs what any other row's does then add those rows
+// unmasked.  (Weierstrass keeps the mask: at z = 0 its term takes the small-|z| branch,
+// which would diverge from the other rows of the same slot.)
 // ---------------------------------------------------------------------------
-template <int NT, int FUNC, bool SKIP = false>
+template <int FUNC>
+constexpr bool zero_term_at_zero() {
+    return FUNC == F_SPHERE || FUNC == F_ELLIPTIC || FUNC == F_ACKLEY || FUNC == F_RASTRIGIN ||
+           FUNC == F_GRIEWANK || FUNC == F_SCHWEFEL || FUNC == F_REF_RASTRIGIN;
+}
+
+template <int NT, int FUNC, bool SKIP = false, bool ZPAD = false>
 __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const float* tab, int h) {
+    // the row mask's bound: with ZPAD one no row reaches
+    const int Dm = (ZPAD && zero_term_at_zero<FUNC>()) ? 0x7FFFFFFF : D;
     float s1 = 0.0f, s2 = (FUNC == F_REF_GRIEWANK) ? 1.0f : 0.0f;
     if constexpr (FUNC == F_ROSENBROCK) {
         float nb[NT][4];
@@ -176,42 +189,42 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
                 const float z = acc[jt][r];
                 if constexpr (FUNC == F_SPHERE) {
                     const float v = z * z;
-                    if (j < D) s1 = s1 + v;
+                    if (j < Dm) s1 = s1 + v;
                 } else if constexpr (FUNC == F_ELLIPTIC) {
                     const float zz = z * z;
                     const float v = tab[j] * zz;
-                    if (j < D) s1 = s1 + v;
+                    if (j < Dm) s1 = s1 + v;
                 } else if constexpr (FUNC == F_ACKLEY) {
                     const float v = z * z;
                     const float sq = sinpi_sq(z);
-                    if (j < D) { s1 = s1 + v; s2 = s2 + sq; }
+                    if (j < Dm) { s1 = s1 + v; s2 = s2 + sq; }
                 } else if constexpr (FUNC == F_RASTRIGIN) {
                     const float zz = z * z;
                     const float s20 = 20.0f * sinpi_sq(z);
                     const float v = zz + s20;
-                    if (j < D) s1 = s1 + v;
+                    if (j < Dm) s1 = s1 + v;
                 } else if constexpr (FUNC == F_GRIEWANK) {
                     const float v = z * z;
                     const float a = 2.0f * sinpi_sq(z * tab[j]);
-                    if (j < D) {
+                    if (j < Dm) {
                         s1 = s1 + v;
                         const float om = 1.0f - s2;
                         const float aq = a * om;
                         s2 = s2 + aq;
                     }
                 } else if constexpr (FUNC == F_WEIERSTRASS) {
-                    if (j < D) s1 = s1 + weier_term(z);
+                    if (j < Dm) s1 = s1 + weier_term(z);
                 } else if constexpr (FUNC == F_SCHWEFEL) {
                     const float sq = sqrtf(__builtin_fabsf(z));
                     const float tv = sq * 0.159154937f;
                     const float v = z * sin2pi(tv);
-                    if (j < D) s1 = s1 + v;
+                    if (j < Dm) s1 = s1 + v;
                 } else if constexpr (FUNC == F_REF_ACKLEY) {
                     // the reference's terms: z^2 and cospif(2 z)
                     const float v = z * z;
                     const float tz = 2.0f * z;
                     const float c = cospi_f(tz);
-                    if (j < D) { s1 = s1 + v; s2 = s2 + c; }
+                    if (j < Dm) { s1 = s1 + v; s2 = s2 + c; }
                 } else if constexpr (FUNC == F_REF_RASTRIGIN) {
                     // z^2 - 10 cospif(2 z) + 10
                     const float zz = z * z;
@@ -220,7 +233,7 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
                     const float t = 10.0f * c;
                     float v = zz - t;
                     v = v + 10.0f;
-                    if (j < D) s1 = s1 + v;
+                    if (j < Dm) s1 = s1 + v;
                 } else if constexpr (FUNC == F_REF_GRIEWANK) {
                     // z^2 and the factor cosf(z / sqrtf(j + 1)), cos q = 1 - 2 sin^2(q / 2)
                     const float v = z * z;
@@ -228,9 +241,9 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
                     const float tq = q * 0.159154937f;
                     const float t = 2.0f * sinpi_sq(tq);
                     const float c = 1.0f - t;
-                    if (j < D) { s1 = s1 + v; s2 = s2 * c; }
+                    if (j < Dm) { s1 = s1 + v; s2 = s2 * c; }
                 } else if constexpr (FUNC == F_REF_WEIERSTRASS) {
-                    if (j < D) s1 = s1 + weier_ref_term(z);
+                    if (j < Dm) s1 = s1 + weier_ref_term(z);
                 }
             }
     }
@@ -433,13 +446,15 @@ __device__ __forceinline__ DeMates de_mates(const EpochArgs& a, const Geo& g, co
 
 // Genes 2 s + h, s = 4 sb .. 4 sb + 3, of a DE/x/1/bin trial: mutant vb + F (v2 - v3)
 // clamped to the box, binomial crossover with the parent vi (16-bit uniforms from
-// block cb: two gene blocks per Philox block), zero past the dimension.
+// block cb: two gene blocks per Philox block).  Slots past the dimension are left as
+// computed (finite, inside the box): every reader of a row takes its first D genes only,
+// and the rotation meets them only in k-steps whose Q entries are zero.
 __device__ __forceinline__ void de_block(float (&t)[4], const float4& vb, const float4& v2, const float4& v3,
-                                         const float4& vi, const U4& cb, int sb, int h, int jrand, int D, float F,
+                                         const float4& vi, const U4& cb, int sb, int h, int jrand, float F,
                                          float lo, float hi, uint32_t tcr) {
-    // gene d = 2 s + h: the tests d == jrand and d < D are made on 2 s against the
-    // lane's jrand - h and D - h, so no per-gene index is held in a register
-    const int jr = jrand - h, dl = D - h;
+    // gene d = 2 s + h: the test d == jrand is made on 2 s against the lane's jrand - h,
+    // so no per-gene index is held in a register
+    const int jr = jrand - h;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int s2 = 2 * (4 * sb + e);
@@ -448,8 +463,7 @@ __device__ __forceinline__ void de_block(float (&t)[4], const float4& vb, const 
         float v = f4(vb, e) + sc;
         v = clamp_box(v, lo, hi);
         const uint32_t u16 = bits16(pick(cb, (uint32_t)(((sb & 1) << 1) | (e >> 1))), e & 1);
-        const float tv = (u16 >= tcr && s2 != jr) ? f4(vi, e) : v;
-        t[e] = (8 * sb + 8 <= D || s2 < dl) ? tv : 0.0f;  // only a block reaching past D is masked
+        t[e] = (u16 >= tcr && s2 != jr) ? f4(vi, e) : v;
     }
 }
 
@@ -583,7 +597,7 @@ __device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs
         } else {
             if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
             de_block(t, lds4(p0 + 4 * sb), lds4(p1 + 4 * sb), lds4(p2 + 4 * sb), lds4(p3 + 4 * sb), cb, sb, h, jrand,
-                     g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
+                     a.p.F, td.lo, td.hi, a.p.cr_threshold);
         }
         float4 q[NT];
 #pragma unroll
@@ -603,8 +617,9 @@ __device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs
     }
     // the padded row groups are skipped for the two costliest epilogues only (measured:
     // the branch costs more than it saves for the others)
-    return objective<NT, FUNC, FUNC == F_WEIERSTRASS || FUNC == F_SCHWEFEL || FUNC == F_REF_WEIERSTRASS>(acc, g.D,
-                                                                                                     L.tab, h);
+    // Q's rows past D are zero in LDS (zeroed at the epoch's start)
+    return objective<NT, FUNC, FUNC == F_WEIERSTRASS || FUNC == F_SCHWEFEL || FUNC == F_REF_WEIERSTRASS, true>(
+        acc, g.D, L.tab, h);
 }
 
 template <int KB>
@@ -1591,7 +1606,7 @@ __device__ __forceinline__ void wide_generation(const EpochArgs& a, const TaskDe
                 }
                 if ((sb & 1) == 0) cb = rng.block((uint32_t)gen, P_DE_CR, (uint32_t)ie, (uint32_t)h, (uint32_t)(sb >> 1));
                 float t[4];
-                de_block(t, vb, v2, v3, vi, cb, sb, h, mt.jrand, g.D, a.p.F, td.lo, td.hi, a.p.cr_threshold);
+                de_block(t, vb, v2, v3, vi, cb, sb, h, mt.jrand, a.p.F, td.lo, td.hi, a.p.cr_threshold);
                 const float4 t4 = make_float4(t[0], t[1], t[2], t[3]);
                 *reinterpret_cast<float4*>(trow + 4 * sb) = t4;
                 mfma_q<NT>(acc, q, sub4(t4, lds4(orow + 4 * sb)));
