@@ -412,11 +412,16 @@ def main():
     best, _ = eng.best()
     eng.close()
 
+    # HBM bytes per step from the committed counter runs (FETCH_SIZE / WRITE_SIZE scaled by
+    # a plain copy) when this is the configuration they were taken on
     traffic = None
     tp = os.path.join(ROOT, "profiles", "island_epoch_traffic.json")
+    tu = os.path.join(ROOT, "profiles", "uneven_epoch_traffic.json")
     if (not uneven and not a.only_func and a.algo == "de" and (a.tasks, a.dim, a.pop, a.interval) == (1024, 50, 128, 20)
             and os.path.exists(tp)):
         traffic = json.load(open(tp))["hbm_bytes_per_launch"]
+    if uneven and a.algo == "de" and (a.tasks, a.interval) == (4096, 20) and os.path.exists(tu):
+        traffic = json.load(open(tu))["hbm_bytes_per_epoch"]
 
     cpu = None
     if a.cpu_gens > 0:
