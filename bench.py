@@ -49,8 +49,8 @@ def parse():
     ap.add_argument("--elites", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-gens", type=int, default=None,
-                    help="generations of the CPU-baseline sample (0 = skip; default: 200 for the plain workload, "
-                         "100 otherwise -- 10 to 40 s of CPU work on 16 threads)")
+                    help="generations of the CPU-baseline sample (0 = skip; default: plain 2000 (GA 400), uneven "
+                         "1000 (GA 200), cvrp 2000 -- 10 to 20 s of CPU work on 16 threads)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
     ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "cvrp", "polish", "online"],
                     help="plain: the rotated continuous workload (default); uneven: 4096 tasks with "
@@ -131,7 +131,7 @@ def bench_cvrp(a):
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         o = oracle.CvrpEngine(inst, **params)
-        g = max(1, a.cpu_gens // 4)
+        g = a.cpu_gens
         t0 = time.perf_counter()
         o.run(g, threads)
         dt = time.perf_counter() - t0
@@ -355,7 +355,8 @@ def main():
               f"GPUs only by design, see the sharding notes)", file=sys.stderr)
         sys.exit(2)
     if a.cpu_gens is None:
-        a.cpu_gens = 200 if a.workload == "plain" else 100
+        ga = a.algo == "ga"
+        a.cpu_gens = {"plain": 400 if ga else 2000, "uneven": 200 if ga else 1000, "cvrp": 2000}.get(a.workload, 100)
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
         return

@@ -107,7 +107,7 @@ enum { EMTF_SCHEDULE_AUTO = 0, EMTF_SCHEDULE_EPOCHS = 1, EMTF_SCHEDULE_PERSISTEN
  * runs `generations` generations from the clock value at its admission, with its own
  * generation counter (the one its draws are addressed by) starting at 0, and transfers draw
  * their donor among the tasks that ran up to the transfer boundary and run on after it --
- * across batches, as the reference draws among every live task (SG-EMTF.cu:1682-1699).
+ * across batches, as the reference draws among every live task (SG-EMTF.cu:1696-1699).
  * ONLINE runs the epochs directly (no graph replay), without autotuning and without polish. */
 enum { EMTF_ADMIT_AT_CREATE = 0, EMTF_ADMIT_ONLINE = 1 };
 

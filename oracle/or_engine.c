@@ -461,10 +461,6 @@ static void snapshot_elites(or_engine* e, int ti) {
         memcpy(s->elites + (size_t)j * D, s->X + (size_t)s->order[j] * D, sizeof(float) * (size_t)D);
 }
 
-/* Transfer into task ti at its generation gen from a donor drawn among the nd tasks of
- * donors[] (ascending ids, ti among them at position pos): the k-th of the others, k = w mod
- * (nd - 1) -- the reference draws rand() % T over every task but itself (SG-EMTF.cu:1696-1699),
- * and over the tasks still running when they arrive online (:1682-1686). */
 /* The donors of the transfer boundary g: the tasks that ran up to it and run on after it
  * (every task of an engine whose tasks all started at 0), ascending ids; returns their count. */
 static int donor_list(const or_engine* e, int g, int* out) {
@@ -480,6 +476,10 @@ static int donor_pick(uint32_t w, int nd, int pos) {
     return kk + (kk >= pos ? 1 : 0);
 }
 
+/* Transfer into task ti at its generation gen from a donor drawn among the nd tasks of
+ * donors[] (ascending ids, ti among them at position pos): the k-th of the others, k = w mod
+ * (nd - 1) -- the reference draws rand() % size over its admitted tasks, again while it hits
+ * itself or a task that has finished (SG-EMTF.cu:1696-1699). */
 static void transfer_task(or_engine* e, int ti, int gen, const int* donors, int nd, int pos, float* work) {
     task_state* s = &e->ts[ti];
     const int D = s->t.dim, P = s->t.pop, k = e->p.transfer_count;

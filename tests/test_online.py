@@ -5,7 +5,7 @@ admitted later runs its own generation counter from 0 (its trajectory is that of
 engine over it alone when nothing transfers); tasks not yet admitted, just admitted or
 finished are never transfer donors; the donor drawn is the k-th other donor in id order,
 k = w mod (n - 1), as the reference draws rand() % T over every other task
-(proposed model/SG-EMTF.cu:1682-1699).
+(proposed model/SG-EMTF.cu:1696-1699).
 GPU: the engine with online admission equals the oracle bit for bit, with transfers
 across batches, batches finishing while others run, and the driver's timed admissions.
 """
