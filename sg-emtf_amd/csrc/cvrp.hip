@@ -22,15 +22,18 @@
 #include "emtf_math.h"
 #include "cvrp_types.h"
 
-#ifndef EMTF_CVRP_SPLIT_BLOCK
-#define EMTF_CVRP_SPLIT_BLOCK 8
-#endif
-
 namespace emtf {
 
 constexpr uint32_t P_CV_INIT = 16, P_CV_GEN = 17;
-constexpr int kRing = 64;  // ring entries kept per lane; routes hold < 64 customers
-constexpr int kSplitBlock = EMTF_CVRP_SPLIT_BLOCK;  // ring entries loaded per round of the window scan
+constexpr int kRing = 64;  // ring positions kept per lane; routes hold < 64 customers
+constexpr int kMirror = 16; // slots 0 .. 15 repeated at 64 .. 79, so any 16 consecutive positions are contiguous
+
+// Lane stride of the split ring in entries: at least kRing + kMirror, and an odd number of
+// 4-byte words, so the 64 lanes of a wave reading the same slot hit 64 different banks.
+template <typename RingT>
+__host__ __device__ constexpr int ring_stride() {
+    return sizeof(RingT) == 2 ? 82 : 81;
+}
 
 // RingT: uint16_t entries (W + woff) when every instance's W + woff range fits 16 bits,
 // else int32_t (W itself, woff = 0).
@@ -41,7 +44,7 @@ struct CvLds {
     uint32_t* xy;   // S node coordinates relative to the depot, (x & 0xFFFF) | (y << 16)
     uint8_t* d0;    // DS depot distances d(0, c)
     uint8_t* dem;   // DS demands
-    RingT* R;       // kRing x popP split ring
+    RingT* R;       // popP x ring_stride split ring (lane-major)
     uint32_t* used; // 8 x popP OX bitmaps
     int32_t* cost;  // popP
     int32_t* ccost; // popP
@@ -51,15 +54,21 @@ struct CvLds {
 
 __host__ __device__ inline int cvrp_lds_bytes(int n, int popP, int ring_bytes) {
     const int NP = (n + 3) & ~3, DS = (n + 1 + 3) & ~3;
-    return kRing * popP * ring_bytes + 8 * popP * 4 + 4 * popP * 4 + 4 * DS + 2 * DS + 2 * popP * NP;
+    const int ring = ring_bytes == 2 ? ring_stride<uint16_t>() * 2 : ring_stride<int32_t>() * 4;
+    return popP * ring + 8 * popP * 4 + 4 * popP * 4 + 4 * DS + 2 * DS + 2 * popP * NP;
 }
 
-// TSPLIB EUC_2D distance nint(sqrt(dx^2 + dy^2)) of two packed relative coordinates
+// TSPLIB EUC_2D distance nint(sqrt(dx^2 + dy^2)) of two packed relative coordinates.
+// Every distance of an instance is <= 255 (checked at create), so s = dx^2 + dy^2 < 255.5^2,
+// and sqrt(s) of an integer s lies at least 0.25 / 511 = 4.9e-4 from every half-integer;
+// the hardware square root (v_sqrt_f32, 1 ulp <= 2^-16 below 256) and the rounding of
+// the + 0.5 stay far inside that, so the rounded distance is exact without the
+// correctly rounded square root's refinement.
 __device__ __forceinline__ int nint_xy(uint32_t a, uint32_t b) {
     const int dx = (int)(int16_t)(a & 0xFFFFu) - (int)(int16_t)(b & 0xFFFFu);
     const int dy = (int)(int16_t)(a >> 16) - (int)(int16_t)(b >> 16);
     const float s = (float)(dx * dx + dy * dy);
-    return (int)(__builtin_sqrtf(s) + 0.5f);
+    return (int)(__builtin_amdgcn_sqrtf(s) + 0.5f);
 }
 
 // Prins split of the lane's tour t (LDS row) as a sliding-window minimum.  With path
@@ -69,57 +78,86 @@ __device__ __forceinline__ int nint_xy(uint32_t a, uint32_t b) {
 // over the route starts i in [lo, j) whose demand fits the capacity (V_0 = 0) -- the same
 // integer sums and minimum as the pull form, so the same labels.  The window start lo
 // follows j (the demand of t_lo .. t_{j-1} kept in a register).  The lane's ring holds
-// W_i + woff for the last kRing positions (one slot per position, shared by the wave, so
-// conflict-free); it starts full of the largest value and every entry is overwritten
-// with it when it leaves the window, so the minimum over the newest 16 slots needs no
-// mask (older or stale slots hold the largest value or, aliased by the ring, a value of
-// the window) -- windows longer than 16 continue in blocks of 8.
+// W_i + woff for the last kRing positions (position i in slot i mod 64, and slots 0 .. 15
+// again in 64 .. 79, so the newest 16 positions sit in 16 consecutive slots read at
+// immediate offsets from one address); the slots of positions -16 .. -1 start at the
+// largest value and every entry is overwritten with it when it leaves the window, so the
+// minimum over the newest 16 positions needs no mask (older slots hold the largest value
+// or, aliased by the ring, a value of the window) -- windows longer than 16 continue in
+// blocks of 8.
 template <typename RingT>
 __device__ __forceinline__ int32_t split_cost(const uint8_t* t, int n, const CvLds<RingT>& L, int cap, int lane,
                                               int popP, int woff) {
     constexpr RingT NONE = (RingT)((sizeof(RingT) == 2) ? 0xFFFF : 0x7FFFFFFF);
-    RingT* R = L.R + lane;
+    constexpr int kS = ring_stride<RingT>();
+    RingT* R = L.R + lane * kS;
+    // every position >= 0 is written (value, or the largest value once it leaves the
+    // window) before any read of it, and reads reach back to position -16 only (the
+    // newest 16 from j = 1 on; the blocks from lo - 7 >= -7): slots 48 .. 63 hold the
+    // negative positions, and they alone start at the largest value
 #pragma unroll
-    for (int k = 0; k < kRing; ++k) R[k * popP] = NONE;
-    int cur = t[0];
+    for (int k = kRing - 16; k < kRing; ++k) R[k] = NONE;
+    // writes slot s and its mirror (the same slot again when s >= 16)
+    auto put = [&](int pos, RingT v) {
+        const int s = pos & (kRing - 1);
+        R[s] = v;
+        R[s < kMirror ? s + kRing : s] = v;
+    };
+    const int cur = t[0];
     uint32_t xyc = L.xy[cur];
     int d0c = L.d0[cur];   // d(0, t_{j-1}) = d(t_{j-1}, 0)
     int dcur = L.dem[cur];
     int32_t P = 0;         // P_{j-1}
     int lo = 0, wload = 0; // window start, demand of t_lo .. t_{j-1}
-    R[0] = (RingT)(d0c + woff);  // W_0 = d(0, t_0)
+    // loaded one step or one window move ahead, so neither waits on its own load: the
+    // customer t_j, the demands of t_lo and t_{lo+1}, the customer t_{lo+2} (clamped to the
+    // tour; a clamped one is never reached, since lo <= n - 1)
+    int nxt = n > 1 ? t[1] : 0;
+    int dlo = dcur, dlo1 = n > 1 ? L.dem[t[1]] : 0;
+    int tl2 = t[n > 2 ? 2 : n - 1];
+    put(0, (RingT)(d0c + woff));  // W_0 = d(0, t_0)
     int32_t V = 0;
     for (int j = 1; j <= n; ++j) {
         // the next customer's coordinates, distances and demand do not depend on this label
-        const int nxt = (j < n) ? t[j] : 0;
+        const int nn = (j + 1 < n) ? t[j + 1] : 0;
         const uint32_t xyn = L.xy[nxt];
         const int d0n = L.d0[nxt];
         const int dmn = L.dem[nxt];
         const int dnx = nint_xy(xyc, xyn);
         wload += dcur;
         while (wload > cap) {  // lo <= j - 1: every demand fits the capacity
-            R[(lo & (kRing - 1)) * popP] = NONE;
-            wload -= L.dem[t[lo]];
+            put(lo, NONE);
+            wload -= dlo;
+            dlo = dlo1;
+            dlo1 = L.dem[tl2];
             ++lo;
+            tl2 = t[lo + 2 < n ? lo + 2 : n - 1];
         }
-        int32_t m = (int32_t)NONE;
+        // positions j - 16 .. j - 1: slots s0 .. s0 + 15 with s0 = (j - 16) mod 64
+        const RingT* Rw = R + ((j - 16) & (kRing - 1));
+        int32_t w[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int32_t w = (int32_t)R[((j - 1 - k) & (kRing - 1)) * popP];
-            m = w < m ? w : m;
+        for (int k = 0; k < 16; ++k) w[k] = (int32_t)Rw[k];
+#pragma unroll
+        for (int h = 8; h >= 1; h >>= 1) {
+#pragma unroll
+            for (int k = 0; k < h; ++k) w[k] = w[k + h] < w[k] ? w[k + h] : w[k];
         }
+        int32_t m = w[0];
         for (int i0 = j - 17; i0 >= lo; i0 -= 8) {
+            // positions i0 - 7 .. i0: slots s1 .. s1 + 7 with s1 = (i0 - 7) mod 64
+            const RingT* Rb = R + ((i0 - 7) & (kRing - 1));
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const int32_t w = (int32_t)R[((i0 - k) & (kRing - 1)) * popP];
-                m = w < m ? w : m;
+                const int32_t v = (int32_t)Rb[k];
+                m = v < m ? v : m;
             }
         }
         V = P + d0c + (m - woff);
         if (j == n) break;
         P += dnx;
-        R[(j & (kRing - 1)) * popP] = (RingT)(V + d0n - P + woff);
-        cur = nxt;
+        put(j, (RingT)(V + d0n - P + woff));
+        nxt = nn;
         xyc = xyn;
         d0c = d0n;
         dcur = dmn;
@@ -203,7 +241,7 @@ __global__ __launch_bounds__(256) void cvrp_epoch_kernel(CvrpArgs a, const int* 
     CvLds<RingT> L;
     uint8_t* p = cv_smem;
     L.R = reinterpret_cast<RingT*>(p);
-    p += kRing * popP * sizeof(RingT);
+    p += ring_stride<RingT>() * popP * sizeof(RingT);
     L.used = reinterpret_cast<uint32_t*>(p);
     p += 8 * popP * 4;
     L.cost = reinterpret_cast<int32_t*>(p);
