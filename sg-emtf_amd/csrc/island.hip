@@ -142,6 +142,103 @@ constexpr bool zero_term_at_zero() {
            FUNC == F_GRIEWANK || FUNC == F_SCHWEFEL || FUNC == F_REF_RASTRIGIN;
 }
 
+// The objective's terms and final combination in split form, for the polish's line search
+// (terms of a row tile computed apart from the running sums): the same operations, in the
+// same order, as objective() below, which keeps them inline for the epoch kernels' code.
+//
+// Rosenbrock's term of row j (z = z_j, zn = z_{j+1}); it counts when row j + 1 exists.
+__device__ __forceinline__ float rosen_val(float z, float zn) {
+    const float zz = z * z;
+    const float a = zz - zn;
+    const float b = z - 1.0f;
+    float t1 = 100.0f * a;
+    t1 = t1 * a;
+    const float b2 = b * b;
+    return t1 + b2;
+}
+
+// The term values of row j of every textbook objective but Rosenbrock (v2: Ackley's
+// cosine term, Griewank's 2 sin^2); the Weierstrass term only for rows j < Dm (0 otherwise).
+template <int FUNC>
+__device__ __forceinline__ void row_vals(float z, int j, int Dm, const float* tab, float& v1, float& v2) {
+    static_assert(FUNC <= F_ELLIPTIC, "textbook objectives only");
+    if constexpr (FUNC == F_SPHERE) {
+        v1 = z * z;
+    } else if constexpr (FUNC == F_ELLIPTIC) {
+        const float zz = z * z;
+        v1 = tab[j] * zz;
+    } else if constexpr (FUNC == F_ACKLEY) {
+        v1 = z * z;
+        v2 = sinpi_sq(z);
+    } else if constexpr (FUNC == F_RASTRIGIN) {
+        const float zz = z * z;
+        const float s20 = 20.0f * sinpi_sq(z);
+        v1 = zz + s20;
+    } else if constexpr (FUNC == F_GRIEWANK) {
+        v1 = z * z;
+        v2 = 2.0f * sinpi_sq(z * tab[j]);
+    } else if constexpr (FUNC == F_WEIERSTRASS) {
+        v1 = 0.0f;
+        if (j < Dm) v1 = weier_term(z);
+    } else if constexpr (FUNC == F_SCHWEFEL) {
+        const float sq = sqrtf(__builtin_fabsf(z));
+        const float tv = sq * 0.159154937f;
+        v1 = z * sin2pi(tv);
+    }
+}
+
+// Adds one row's term values into the lane's sums (s2: Ackley's cosine sum, Griewank's
+// running 1 - prod cos) when the row counts.
+template <int FUNC>
+__device__ __forceinline__ void row_sum(float v1, float v2, bool in, float& s1, float& s2) {
+    if (!in) return;
+    s1 = s1 + v1;
+    if constexpr (FUNC == F_ACKLEY) {
+        s2 = s2 + v2;
+    } else if constexpr (FUNC == F_GRIEWANK) {
+        const float om = 1.0f - s2;
+        const float aq = v2 * om;
+        s2 = s2 + aq;
+    }
+}
+
+// The fitness from the two halves' sums (lane h holds its half's s1, s2; the other
+// half's come over the cross-half shuffle), textbook objectives.
+template <int FUNC>
+__device__ __forceinline__ float objective_tail(float s1, float s2, int D, const float* tab, int h) {
+    static_assert(FUNC <= F_ELLIPTIC, "textbook objectives only");
+    const float o1 = __shfl_xor(s1, 32);
+    const float S = s1 + o1;  // A + B (commutative)
+    const float Df = (float)D;
+    if constexpr (FUNC == F_ACKLEY) {
+        const float o2 = __shfl_xor(s2, 32);
+        const float C = s2 + o2;
+        const float q = S / Df;
+        const float r = sqrtf(q);
+        const float a = -0.2f * r;
+        const float t1 = -20.0f * expm1_f(a);
+        const float c2 = C / Df;
+        const float b = -2.0f * c2;
+        const float t2 = 2.71828175f * expm1_f(b);
+        return t1 - t2;
+    } else if constexpr (FUNC == F_GRIEWANK) {
+        const float o2 = __shfl_xor(s2, 32);
+        const float qA = h ? o2 : s2;
+        const float qB = h ? s2 : o2;
+        const float om = 1.0f - qA;
+        const float tq = qB * om;
+        const float Qv = qA + tq;
+        const float s4 = S / 4000.0f;
+        return s4 + Qv;
+    } else if constexpr (FUNC == F_SCHWEFEL) {
+        // the constant from the task table: 418.982887 (textbook) or 418.9829 (reference)
+        const float cd = tab[0] * Df;
+        return cd - S;
+    } else {
+        return S;
+    }
+}
+
 template <int NT, int FUNC, bool SKIP = false, bool ZPAD = false>
 __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const float* tab, int h) {
     // the row mask's bound: with ZPAD one no row reaches
@@ -301,6 +398,51 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
         return S - dw;
     } else {
         return S;
+    }
+}
+
+// One 32-row tile (tile jt, accumulators acc) of the objective, in two parts: the row
+// terms (independent of every other tile; zn0: Rosenbrock's z of row 32 (jt + 1) for this
+// column, 0 past the last tile), then their sums onto the lane's running s1, s2 -- the
+// rows of tile jt in the order objective() adds them.
+template <int FUNC>
+__device__ __forceinline__ void tile_vals(const f16v& acc, int jt, int D, const float* tab, int h, float zn0,
+                                          float (&v1)[16], float (&v2)[16]) {
+    if constexpr (FUNC == F_ROSENBROCK) {
+        float nb[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) nb[g] = __shfl_xor(acc[4 * g], 32);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int g = r >> 2;
+            float zn;
+            if ((r & 3) != 3) zn = acc[r + 1];
+            else if (g < 3) zn = h ? nb[g + 1] : nb[g];
+            else zn = h ? zn0 : nb[g];
+            v1[r] = rosen_val(acc[r], zn);
+            v2[r] = 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            v1[r] = 0.0f;
+            v2[r] = 0.0f;
+            row_vals<FUNC>(acc[r], 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3), D, tab, v1[r], v2[r]);
+        }
+    }
+}
+
+template <int FUNC>
+__device__ __forceinline__ void tile_sums(const float (&v1)[16], const float (&v2)[16], int jt, int D, int h, float& s1,
+                                          float& s2) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int j = 32 * jt + 8 * (r >> 2) + 4 * h + (r & 3);
+        if constexpr (FUNC == F_ROSENBROCK) {
+            if (j + 1 < D) s1 = s1 + v1[r];
+        } else {
+            row_sum<FUNC>(v1[r], v2[r], j < D, s1, s2);
+        }
     }
 }
 
@@ -961,11 +1103,12 @@ __device__ __forceinline__ void dfdz(float (&g)[4 * KB], const float* z, float* 
 
 // Polish staging per elite slot (floats): the point in the [even | odd] row layout, z,
 // dz, the gradient and Griewank's prefix / suffix products (natural order), scalars,
-// and the line search's candidates of one round (fitness, step) -- 4 waves x cpe.
+// and the line search's candidates of one round (fitness, step) -- cpe each.
 template <int KB>
 struct PolishSlot {
     static constexpr int RS = Shape<KB>::RS, ZS = ObjShape<KB>::ZS;
-    __host__ __device__ static constexpr int floats(int cpe) { return RS + 5 * ZS + 16 + 8 * cpe; }
+    __host__ __device__ static constexpr int floats(int cpe) { return RS + 5 * ZS + 16 + 2 * cpe; }
+    static constexpr int kScratch = 128 + 128;  // after the slots: line-search sums, Rosenbrock rows
 };
 
 // Armijo polish of the top-k elites of one task, run between epochs after generation g1
@@ -973,11 +1116,12 @@ struct PolishSlot {
 // halve alpha until f(clamp(x - alpha g)) <= f - 1e-4 alpha |g|^2 (at most 24 halvings),
 // take the step, repeat.  The elites go through in groups of 32 / cpe slots, all four
 // waves working on every step of a group: each wave computes one 32-row tile of
-// z = Q (x - o) and of g = Q^T dz for every slot (column c holds slot c / cpe); dz is
-// spread over all threads by (slot, dimension); the line search evaluates 4 x cpe
-// halvings per slot at once (wave w, column sub: halving 4-wave round base + w cpe + sub),
-// and the smallest accepted halving wins -- the step the one-at-a-time search takes.
-// At a transfer boundary the task's elites are published afterwards.
+// z = Q (x - o) (first step only) and of g = Q^T dz for every slot (column c holds slot
+// c / cpe); dz is spread over all threads by (slot, dimension); the line search evaluates
+// cpe halvings per slot per round (column c: halving round base + c mod cpe), each wave
+// one row tile of the candidates' z, and the smallest accepted halving wins -- the step
+// the one-at-a-time search takes; its z is kept as the next step's.  At a transfer
+// boundary the task's elites are published afterwards.
 template <int KB, int FUNC>
 __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, const TaskDev& td) {
     using S = Shape<KB>;
@@ -997,6 +1141,9 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
     L.ord = reinterpret_cast<int*>(L.fit + g.popP);
     const int cpe = a.polish_cpe, nsl = 32 / cpe, PSZ = PolishSlot<KB>::floats(cpe);
     float* const slots = reinterpret_cast<float*>(L.ord + g.popP);
+    float* const PS = slots + nsl * PSZ;  // the line search's running row sums, s1 | s2 per lane
+    float* const PZ = PS + 128;           // Rosenbrock: row 0 of each tile per candidate
+    static_assert(NT <= NW, "one row tile per wave in the line search");
     const int nz = 32 * NT * RS + RS + 32 * NT;
     for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
     __syncthreads();
@@ -1026,7 +1173,7 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
     auto SF = [&](int sl) { return SUF(sl) + ZS; };  // f, g2, alpha0, ka, kb, fnew, alpha_ok
     auto SI = [&](int sl) { return reinterpret_cast<int*>(SF(sl) + 8); };  // active, done, first, elite
     auto CF = [&](int sl) { return SF(sl) + 16; };
-    auto CA = [&](int sl) { return CF(sl) + NW * cpe; };
+    auto CA = [&](int sl) { return CF(sl) + cpe; };
     constexpr int kNone = 1 << 30;
     const int my_sl = col / cpe, sub = col - my_sl * cpe;
     for (int gb = 0; gb < K; gb += nsl) {
@@ -1045,8 +1192,9 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
         }
         __syncthreads();
         for (int step = 0; step < a.p.polish_steps; ++step) {
-            // z = Q (x - o) of every slot, one 32-row tile per wave
-            for (int jt = wave; jt < NT; jt += NW) {
+            // z = Q (x - o) of every slot, one 32-row tile per wave (after the first step the
+            // line search left it: an accepted slot's z, and an inactive slot's is unused)
+            for (int jt = wave; step == 0 && jt < NT; jt += NW) {
                 const float* xr = X(my_sl) + h * HS;
                 f16v acc;
 #pragma unroll
@@ -1132,27 +1280,62 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
                 SI(sl)[2] = kNone;
             }
             __syncthreads();
-            for (int r0 = 0;; r0 += NW * cpe) {
-                const int b = r0 + wave * cpe + sub;
+            // Armijo rounds: halvings r0 .. r0 + cpe - 1 of every slot (column col: slot
+            // col / cpe, halving r0 + col % cpe).  Wave w computes row tile w of the 32
+            // candidates' z, and the objective's row sums run through the tiles in order:
+            // wave w adds tile w's rows onto the sums wave w - 1 left in LDS, the last wave
+            // finishes the fitness.  The accepted candidate's z is the next step's z.
+            for (int r0 = 0;; r0 += cpe) {
+                const int b = r0 + sub;
                 const bool act = my_sl < ns && SI(my_sl)[0] && !SI(my_sl)[1];
                 float alpha = SF(my_sl)[2];
                 for (int q = 0; q < b; ++q) alpha = alpha * 0.5f;
                 const float* xr = X(my_sl) + h * HS;
                 const float* gr = G(my_sl);
-                float xn[HS];
+                f16v acc;
 #pragma unroll
-                for (int s2 = 0; s2 < HS; ++s2) {
-                    const float st = alpha * gr[2 * s2 + h];
-                    xn[s2] = (2 * s2 + h < g.D) ? clampf(xr[s2] - st, td.lo, td.hi) : 0.0f;
+                for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+                if (wave < NT) {
+                    const float* qrow = L.Q + (32 * wave + col) * RS + h * HS;
+                    for (int s2 = 0; s2 < g.Dh; ++s2) {
+                        const float st = alpha * gr[2 * s2 + h];
+                        const float xn = (2 * s2 + h < g.D) ? clampf(xr[s2] - st, td.lo, td.hi) : 0.0f;
+                        const float w = xn - L.o[h * HS + s2];
+                        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qrow[s2], w, acc, 0, 0, 0);
+                    }
+                    if (FUNC == F_ROSENBROCK && h == 0) PZ[32 * wave + col] = acc[0];
                 }
-                const float fn = evaluate_tile<KB, FUNC>(xn, g, L, col, h);
-                const float c1 = 1.0e-4f * alpha;
-                const float c2 = c1 * SF(my_sl)[1];
-                const float thr = SF(my_sl)[0] - c2;
-                if (act && b < 24 && fn <= thr && h == 0) {
-                    CF(my_sl)[b - r0] = fn;
-                    CA(my_sl)[b - r0] = alpha;
-                    atomicMin(&SI(my_sl)[2], b);
+                __syncthreads();
+                float v1[16], v2[16];
+                if (wave < NT) {
+                    const float zn0 = (FUNC == F_ROSENBROCK && wave + 1 < NT) ? PZ[32 * (wave + 1) + col] : 0.0f;
+                    tile_vals<FUNC>(acc, wave, g.D, L.tab, h, zn0, v1, v2);
+                }
+                float s1 = 0.0f, s2 = 0.0f;
+                for (int st = 0; st < NT; ++st) {
+                    if (wave == st) {
+                        if (st > 0) {
+                            s1 = PS[lane];
+                            s2 = PS[64 + lane];
+                        }
+                        tile_sums<FUNC>(v1, v2, st, g.D, h, s1, s2);
+                        if (st + 1 < NT) {
+                            PS[lane] = s1;
+                            PS[64 + lane] = s2;
+                        }
+                    }
+                    if (st + 1 < NT) __syncthreads();
+                }
+                if (wave == NT - 1) {
+                    const float fn = objective_tail<FUNC>(s1, s2, g.D, L.tab, h);
+                    const float c1 = 1.0e-4f * alpha;
+                    const float c2 = c1 * SF(my_sl)[1];
+                    const float thr = SF(my_sl)[0] - c2;
+                    if (act && b < 24 && fn <= thr && h == 0) {
+                        CF(my_sl)[sub] = fn;
+                        CA(my_sl)[sub] = alpha;
+                        atomicMin(&SI(my_sl)[2], b);
+                    }
                 }
                 __syncthreads();
                 for (int sl = tid; sl < ns; sl += kThreads) {
@@ -1164,9 +1347,15 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
                     }
                 }
                 __syncthreads();
+                // the accepted halving's lanes keep z = Q (x_new - o) for the next step (the
+                // same MFMA chain the forward pass would run on the stepped point)
+                if (wave < NT && my_sl < ns && SI(my_sl)[2] == b) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) Z(my_sl)[32 * wave + 8 * (r >> 2) + 4 * h + (r & 3)] = acc[r];
+                }
                 bool more = false;
                 for (int sl = 0; sl < ns; ++sl) more = more || (SI(sl)[0] && !SI(sl)[1]);
-                if (!more || r0 + NW * cpe >= 24) break;
+                if (!more || r0 + cpe >= 24) break;
             }
             // accepted slots step to clamp(x - alpha g), the others stop
             for (int e = tid; e < ns * 2 * HS; e += kThreads) {
@@ -2015,7 +2204,8 @@ extern "C" int EMTF_OBJ_NAME(emtf_polish_lds_floats_kb, EMTF_KB)(int pop, int wa
     using S = emtf::Shape<EMTF_KB>;
     (void)waves;
     const int popP = (pop + 31) & ~31;
-    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * popP + (32 / cpe) * emtf::PolishSlot<EMTF_KB>::floats(cpe);
+    return 32 * S::NT * S::RS + S::RS + 32 * S::NT + 2 * popP + (32 / cpe) * emtf::PolishSlot<EMTF_KB>::floats(cpe) +
+           emtf::PolishSlot<EMTF_KB>::kScratch;
 }
 
 extern "C" int EMTF_OBJ_NAME(emtf_launch_polish_kb, EMTF_KB)(const emtf::EpochArgs* args, const int* order, int n,
