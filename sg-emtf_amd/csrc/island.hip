@@ -68,10 +68,11 @@ __host__ __device__ inline int lds_floats_kb(int pop, int kb) {
 __device__ __forceinline__ float4 lds4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 __device__ __forceinline__ float f4(const float4& v, int e) { return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w)); }
 
+// v clamped to [lo, hi] (hi if v > hi, else lo if v < lo, else v: a NaN stays NaN), as
+// two selects.
 __device__ __forceinline__ float clampf(float v, float lo, float hi) {
-    if (v > hi) v = hi;
-    else if (v < lo) v = lo;
-    return v;
+    const float t = v < lo ? lo : v;
+    return v > hi ? hi : t;
 }
 
 // The box clamp of a trial gene as one min and one max: equal to clampf for every
@@ -764,6 +765,30 @@ __device__ __forceinline__ float trial_eval(float (&tr)[4 * KB], const EpochArgs
         acc, g.D, L.tab, h);
 }
 
+// Copies an n x D row-major matrix from global memory into LDS rows of stride RS in the
+// [even | odd] gene layout (gene d at (d & 1) HS + d / 2), eight loads in flight per
+// thread, so a load's latency is paid once per eight elements rather than per element.
+template <int RS, int HS>
+__device__ __forceinline__ void stage_rows(float* dst, const float* __restrict__ src, int n, int D, int tid) {
+    const int total = n * D;
+    for (int e0 = tid; e0 < total; e0 += 8 * kThreads) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int e = e0 + k * kThreads;
+            v[k] = e < total ? src[e] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int e = e0 + k * kThreads;
+            if (e < total) {
+                const int j = e / D, d = e - j * D;
+                dst[j * RS + (d & 1) * HS + (d >> 1)] = v[k];
+            }
+        }
+    }
+}
+
 template <int KB>
 __device__ __forceinline__ void load_half_row(float (&x)[4 * KB], const float* row) {
 #pragma unroll
@@ -1000,18 +1025,25 @@ struct DzConst {
     float ka, kb;
 };
 
+// Ackley's gradient factors from its sums s1 = sum z_j^2, s2 = sum sin^2(pi z_j).
+__device__ __forceinline__ DzConst ackley_const(float s1, float s2, int D) {
+    DzConst k;
+    const float Df = (float)D;
+    const float r = sqrtf(s1 / Df);
+    const float e1 = expm1_f(-0.2f * r) + 1.0f;
+    const float e2 = expm1_f(-2.0f * (s2 / Df)) + 1.0f;
+    k.ka = (r > 0.0f) ? (4.0f * e1) / (Df * r) : 0.0f;
+    k.kb = (6.28318548f * 2.71828175f / Df) * e2;
+    return k;
+}
+
 template <int FUNC>
 __device__ __forceinline__ DzConst dz_const(const float* z, int D) {
     DzConst k{0.0f, 0.0f};
     if constexpr (FUNC == F_ACKLEY) {
         float s1 = 0.0f, s2 = 0.0f;
         for (int j = 0; j < D; ++j) { s1 = s1 + z[j] * z[j]; s2 = s2 + sinpi_sq(z[j]); }
-        const float Df = (float)D;
-        const float r = sqrtf(s1 / Df);
-        const float e1 = expm1_f(-0.2f * r) + 1.0f;
-        const float e2 = expm1_f(-2.0f * (s2 / Df)) + 1.0f;
-        k.ka = (r > 0.0f) ? (4.0f * e1) / (Df * r) : 0.0f;
-        k.kb = (6.28318548f * 2.71828175f / Df) * e2;
+        k = ackley_const(s1, s2, D);
     }
     return k;
 }
@@ -1101,13 +1133,34 @@ __device__ __forceinline__ void dfdz(float (&g)[4 * KB], const float* z, float* 
     }
 }
 
+// z rows of tile jt for the lane's column from w = x - o (the lane's parity half, in
+// registers): the k-steps of evaluate_tile's chain for that tile, whole blocks of four
+// from the first, stopping after the block that holds step Dh - 1 (the steps past the
+// dimension in it add zeros).
+template <int KB>
+__device__ __forceinline__ f16v tile_z(const float (&w)[4 * KB], const float* Q, int jt, int Dh, int col, int h) {
+    using S = Shape<KB>;
+    f16v acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    const float* qrow = Q + (32 * jt + col) * S::RS + h * S::HS;
+#pragma unroll
+    for (int sb = 0; sb < KB; ++sb) {
+        if (4 * sb >= Dh) break;
+        const float4 q = lds4(qrow + 4 * sb);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(f4(q, e), w[4 * sb + e], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 // Polish staging per elite slot (floats): the point in the [even | odd] row layout, z,
 // dz, the gradient and Griewank's prefix / suffix products (natural order), scalars,
 // and the line search's candidates of one round (fitness, step) -- cpe each.
 template <int KB>
 struct PolishSlot {
-    static constexpr int RS = Shape<KB>::RS, ZS = ObjShape<KB>::ZS;
-    __host__ __device__ static constexpr int floats(int cpe) { return RS + 5 * ZS + 16 + 2 * cpe; }
+    static constexpr int RS = Shape<KB>::RS, ZS = 32 * Shape<KB>::NT + 4;  // rows 16-byte aligned
+    __host__ __device__ static constexpr int floats(int cpe) { return (RS + 5 * ZS + 16 + 2 * cpe + 3) & ~3; }
     static constexpr int kScratch = 128 + 128;  // after the slots: line-search sums, Rosenbrock rows
 };
 
@@ -1125,7 +1178,7 @@ struct PolishSlot {
 template <int KB, int FUNC>
 __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, const TaskDev& td) {
     using S = Shape<KB>;
-    constexpr int RS = S::RS, HS = S::HS, NT = S::NT, ZS = ObjShape<KB>::ZS;
+    constexpr int RS = S::RS, HS = S::HS, NT = S::NT, ZS = PolishSlot<KB>::ZS;
     constexpr int NW = kThreads / 64;
     const int tid = threadIdx.x, lane = tid & 63, col = lane & 31, h = lane >> 5, wave = tid >> 6;
     Geo g{};
@@ -1147,11 +1200,7 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
     const int nz = 32 * NT * RS + RS + 32 * NT;
     for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
     __syncthreads();
-    const float* Qg = a.Q + td.q_off;
-    for (int e = tid; e < g.D * g.D; e += kThreads) {
-        const int j = e / g.D, d = e - j * g.D;
-        L.Q[j * RS + (d & 1) * HS + (d >> 1)] = Qg[e];
-    }
+    stage_rows<RS, HS>(L.Q, a.Q + td.q_off, g.D, g.D, tid);
     for (int d = tid; d < g.D; d += kThreads) {
         L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
         L.tab[d] = a.tab[td.v_off + d];
@@ -1196,23 +1245,45 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
             // line search left it: an accepted slot's z, and an inactive slot's is unused)
             for (int jt = wave; step == 0 && jt < NT; jt += NW) {
                 const float* xr = X(my_sl) + h * HS;
-                f16v acc;
+                float w[HS];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-                for (int s2 = 0; s2 < g.Dh; ++s2) {
-                    const float w = xr[s2] - L.o[h * HS + s2];
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(L.Q[(32 * jt + col) * RS + h * HS + s2], w, acc, 0, 0, 0);
+                for (int s4 = 0; s4 < HS; s4 += 4) {
+                    const float4 x4 = lds4(xr + s4), o4 = lds4(L.o + h * HS + s4);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) w[s4 + e] = f4(x4, e) - f4(o4, e);
                 }
+                const f16v acc = tile_z<KB>(w, L.Q, jt, g.Dh, col, h);
                 if (sub == 0) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) Z(my_sl)[32 * jt + 8 * (r >> 2) + 4 * h + (r & 3)] = acc[r];
                 }
             }
             __syncthreads();
-            // per-slot factors: Ackley's, Griewank's prefix (even threads) / suffix (odd) products
+            // per-slot factors: Ackley's, Griewank's prefix (even threads) / suffix (odd)
+            // products; their per-dimension terms first, over all (slot, dimension) pairs
+            // (into the dz and prefix rows, free until the dz pass), then the sums and
+            // products in dimension order, one thread per slot and direction
+            if constexpr (FUNC == F_ACKLEY || FUNC == F_GRIEWANK) {
+                for (int e = tid; e < ns * g.D; e += kThreads) {
+                    const int sl = e / g.D, j = e - sl * g.D;
+                    const float zj = Z(sl)[j];
+                    if constexpr (FUNC == F_ACKLEY) {
+                        DZ(sl)[j] = zj * zj;
+                        PRE(sl)[j] = sinpi_sq(zj);
+                    } else {
+                        DZ(sl)[j] = griewank_cos(zj, L.tab[j]);
+                    }
+                }
+                __syncthreads();
+            }
             if constexpr (FUNC == F_ACKLEY) {
                 for (int sl = tid; sl < ns; sl += kThreads) {
-                    const DzConst kc = dz_const<FUNC>(Z(sl), g.D);
+                    float s1 = 0.0f, s2 = 0.0f;
+                    for (int j = 0; j < g.D; ++j) {
+                        s1 = s1 + DZ(sl)[j];
+                        s2 = s2 + PRE(sl)[j];
+                    }
+                    const DzConst kc = ackley_const(s1, s2, g.D);
                     SF(sl)[3] = kc.ka;
                     SF(sl)[4] = kc.kb;
                 }
@@ -1221,17 +1292,17 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
             if constexpr (FUNC == F_GRIEWANK) {
                 for (int e = tid; e < 2 * ns; e += kThreads) {
                     const int sl = e >> 1;
-                    const float* z = Z(sl);
+                    const float* c = DZ(sl);
                     float p = 1.0f;
                     if ((e & 1) == 0) {
                         for (int j = 0; j < g.D; ++j) {
                             PRE(sl)[j] = p;
-                            p = p * griewank_cos(z[j], L.tab[j]);
+                            p = p * c[j];
                         }
                     } else {
                         for (int j = g.D - 1; j >= 0; --j) {
                             SUF(sl)[j] = p;
-                            p = p * griewank_cos(z[j], L.tab[j]);
+                            p = p * c[j];
                         }
                     }
                 }
@@ -1256,10 +1327,21 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
                 f16v gacc;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) gacc[r] = 0.0f;
-                for (int s2 = 0; s2 < g.Dh; ++s2) {
-                    const int j = 2 * s2 + h;
-                    gacc = __builtin_amdgcn_mfma_f32_32x32x2f32(L.Q[j * RS + (d & 1) * HS + (d >> 1)], dzr[j], gacc, 0, 0,
-                                                                0);
+                // blocks of four k-steps (rows j = 2 s2 + h); the steps past the dimension
+                // in the last block add zeros (Q's padding rows and dz past D are zero)
+                const float* qc = L.Q + (d & 1) * HS + (d >> 1);
+#pragma unroll
+                for (int sb = 0; sb < KB; ++sb) {
+                    if (4 * sb >= g.Dh) break;
+                    float qv[4], zv[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int j = 2 * (4 * sb + e) + h;
+                        qv[e] = qc[j * RS];
+                        zv[e] = dzr[j];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) gacc = __builtin_amdgcn_mfma_f32_32x32x2f32(qv[e], zv[e], gacc, 0, 0, 0);
                 }
                 if (sub == 0) {
 #pragma unroll
@@ -1271,7 +1353,15 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
             for (int sl = tid; sl < ns; sl += kThreads) {
                 const float* gr = G(sl);
                 float g2 = 0.0f;
-                for (int d = 0; d < g.D; ++d) g2 = fmaf(gr[d], gr[d], g2);
+                int d = 0;
+                for (; d + 8 <= g.D; d += 8) {  // eight loads in flight, the chain in order
+                    float v[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) v[k] = gr[d + k];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) g2 = fmaf(v[k], v[k], g2);
+                }
+                for (; d < g.D; ++d) g2 = fmaf(gr[d], gr[d], g2);
                 const float f = SF(sl)[0];
                 SF(sl)[1] = g2;
                 SF(sl)[2] = f / g2;
@@ -1296,13 +1386,20 @@ __device__ __forceinline__ void polish_body(const EpochArgs& a, float* smem, con
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
                 if (wave < NT) {
-                    const float* qrow = L.Q + (32 * wave + col) * RS + h * HS;
-                    for (int s2 = 0; s2 < g.Dh; ++s2) {
-                        const float st = alpha * gr[2 * s2 + h];
-                        const float xn = (2 * s2 + h < g.D) ? clampf(xr[s2] - st, td.lo, td.hi) : 0.0f;
-                        const float w = xn - L.o[h * HS + s2];
-                        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(qrow[s2], w, acc, 0, 0, 0);
+                    float w[HS];
+#pragma unroll
+                    for (int s4 = 0; s4 < HS; s4 += 4) {
+                        const float4 x4 = lds4(xr + s4), o4 = lds4(L.o + h * HS + s4);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int s2 = s4 + e;
+                            const float st = alpha * gr[2 * s2 + h];  // in the row (zeros past D)
+                            const float xc = clampf(f4(x4, e) - st, td.lo, td.hi);
+                            const float xn = (2 * s2 + h < g.D) ? xc : 0.0f;
+                            w[s2] = xn - f4(o4, e);
+                        }
                     }
+                    acc = tile_z<KB>(w, L.Q, wave, g.Dh, col, h);
                     if (FUNC == F_ROSENBROCK && h == 0) PZ[32 * wave + col] = acc[0];
                 }
                 __syncthreads();
@@ -1441,11 +1538,7 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
     const int nz = L.fit - smem;  // X, Q, o, tab
     for (int e = tid; e < nz; e += kThreads) smem[e] = 0.0f;
     __syncthreads();
-    const float* Qg = a.Q + td.q_off;
-    for (int e = tid; e < g.D * g.D; e += kThreads) {
-        const int j = e / g.D, d = e - j * g.D;
-        L.Q[j * RS + (d & 1) * HS + (d >> 1)] = Qg[e];
-    }
+    stage_rows<RS, HS>(L.Q, a.Q + td.q_off, g.D, g.D, tid);
     for (int d = tid; d < g.D; d += kThreads) {
         L.o[(d & 1) * HS + (d >> 1)] = a.shift[td.v_off + d];
         L.tab[d] = a.tab[td.v_off + d];
@@ -1467,11 +1560,7 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
             if (i < g.pop && h == 0) L.fit[i] = f;
         }
     } else {
-        const float* Xg = a.X + td.x_off;
-        for (int e = tid; e < g.pop * g.D; e += kThreads) {
-            const int i = e / g.D, d = e - i * g.D;
-            L.X[i * RS + (d & 1) * HS + (d >> 1)] = Xg[e];
-        }
+        stage_rows<RS, HS>(L.X, a.X + td.x_off, g.pop, g.D, tid);
         for (int i = tid; i < g.pop; i += kThreads) L.fit[i] = a.fit[td.fit_off + i];
     }
     __syncthreads();
