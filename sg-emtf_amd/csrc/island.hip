@@ -2003,9 +2003,22 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     if constexpr (QL) {
         L.Q = smem;
         base = smem + 32 * NT * RS;
-        const float* qg = a.Qw + td.qw_off;
-        for (int e = tid; e < 32 * NT * RS / 4; e += WT)
-            reinterpret_cast<float4*>(smem)[e] = reinterpret_cast<const float4*>(qg)[e];
+        // eight 16-byte loads in flight per thread
+        const float4* qg = reinterpret_cast<const float4*>(a.Qw + td.qw_off);
+        constexpr int n4 = 32 * NT * RS / 4;
+        for (int e0 = tid; e0 < n4; e0 += 8 * WT) {
+            float4 v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int e = e0 + k * WT;
+                v[k] = qg[e < n4 ? e : n4 - 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int e = e0 + k * WT;
+                if (e < n4) reinterpret_cast<float4*>(smem)[e] = v[k];
+            }
+        }
     } else {
         L.Q = const_cast<float*>(a.Qw + td.qw_off);
     }
@@ -2027,20 +2040,37 @@ __device__ __forceinline__ void wide_body(const EpochArgs& a, float* smem, const
     // rows in the padded layout, zero past the dimension
     const float* Xg = a.X + td.x_off;
     const float lo = td.lo, rng_ = td.hi - td.lo;
-    for (int e = tid; e < g.popP * RS; e += WT) {
-        const int i = e / RS, c = e - i * RS;
-        const int d = (c < HS) ? 2 * c : 2 * (c - HS) + 1;
-        float v = 0.0f;
-        if (i < g.pop && c < 2 * HS && d < g.D) {
-            if (a.init) {
+    const int nx = g.popP * RS;
+    if (a.init) {
+        for (int e = tid; e < nx; e += WT) {
+            const int i = e / RS, c = e - i * RS;
+            const int d = (c < HS) ? 2 * c : 2 * (c - HS) + 1;
+            float v = 0.0f;
+            if (i < g.pop && c < 2 * HS && d < g.D) {
                 const float u = uniform01(rng.word(0u, P_INIT, (uint32_t)i, 0u, (uint32_t)d));
                 const float t1 = u * rng_;
                 v = t1 + lo;
-            } else {
-                v = Xg[(int64_t)i * g.D + d];
+            }
+            L.X[e] = v;
+        }
+    } else {
+        for (int e0 = tid; e0 < nx; e0 += 8 * WT) {  // eight loads in flight per thread
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int e = e0 + k * WT;
+                const int i = e / RS, c = e - i * RS;
+                const int d = (c < HS) ? 2 * c : 2 * (c - HS) + 1;
+                const bool in = e < nx && i < g.pop && c < 2 * HS && d < g.D;
+                const float xv = Xg[in ? (int64_t)i * g.D + d : 0];
+                v[k] = in ? xv : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int e = e0 + k * WT;
+                if (e < nx) L.X[e] = v[k];
             }
         }
-        L.X[e] = v;
     }
     if (!a.init)
         for (int i = tid; i < g.pop; i += WT) L.fit[i] = a.fit[td.fit_off + i];
