@@ -240,7 +240,8 @@ def bench_polish(a):
     transferred every 20 generations; the polished variant adds 5 Armijo steps on the top 4
     elites of every task every 50 generations.  Error = float64 re-evaluation of each task's
     best point (Schwefel excluded: no known minimum inside the box).  Reports the GPU time to
-    mean error <= 1e-6 of both variants over seeds 0..N-1 and the per-objective picture."""
+    mean error <= 1e-6 of both variants over seeds 0..N-1 and the per-objective picture
+    (--only-func: every task of that objective, for per-objective kernel timing)."""
     import numpy as np
     import emtf
     from emtf import tasks as tg
@@ -251,7 +252,8 @@ def bench_polish(a):
     per_func = {m: {n: [] for n in names} for m in res}
     final = {m: [] for m in res}
     for seed in range(a.seeds):
-        tasks = tg.benchmark_tasks(1024, 100, 128, rotate=True, seed=seed)
+        only = [emtf.FUNC_IDS[a.only_func]] * 1024 if a.only_func else None
+        tasks = tg.benchmark_tasks(1024, 100, 128, rotate=True, seed=seed, funcs=only)
         funcs = np.array([t["func"] for t in tasks])
         for mode in res:
             p = dict(base, seed=seed)
@@ -259,7 +261,7 @@ def bench_polish(a):
                 p.update(polish_interval=50, polish_steps=5, polish_topk=4)
             t_ms, errs = _converge(tasks, p, a.gens, chunk, target)
             live = funcs != 6
-            mean_err = np.nanmean(errs[:, live], axis=1)
+            mean_err = np.nanmean(errs[:, live], axis=1) if live.any() else np.full(errs.shape[0], np.nan)
             res[mode].append(_time_to(t_ms, mean_err, target))
             final[mode].append((float(mean_err[-1]), float(t_ms[-1])))
             for f, n in enumerate(names):
