@@ -69,8 +69,11 @@ def parse():
                     help="only run a plain device copy of MIB MiB (reference for HBM counters)")
     ap.add_argument("--no-study", action="store_true",
                     help="plain workload: skip the one-seed time-to-1e-6 study (adjoint polish vs plain DE)")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "epochs"],
-                    help="auto: epochs replayed from a captured hipGraph; epochs: launched directly")
+    ap.add_argument("--one-call", action="store_true",
+                    help="run the timed steps as one emtf_run call (the persistent schedule's single launch)")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "epochs", "persistent"],
+                    help="auto: epochs replayed from a captured hipGraph; epochs: launched directly; "
+                         "persistent: one launch per run, epochs ordered on the device")
     return ap.parse_args()
 
 
@@ -382,7 +385,8 @@ def main():
     else:
         funcs = [emtf.FUNC_IDS[a.only_func]] * a.tasks if a.only_func else None
         tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=True, seed=a.seed, funcs=funcs)
-    sched = emtf.SCHEDULE_AUTO if a.schedule == "auto" else emtf.SCHEDULE_EPOCHS
+    sched = {"auto": emtf.SCHEDULE_AUTO, "epochs": emtf.SCHEDULE_EPOCHS,
+             "persistent": emtf.SCHEDULE_PERSISTENT}[a.schedule]
     if a.algo == "ga":
         params = dict(algo=emtf.GA_SBX_PM, eta_c=2.0, eta_m=5.0, pm=1.0, transfer_interval=a.interval,
                       transfer_count=a.elites, seed=a.seed, tune=1 if a.no_tune else 0, schedule=sched)
@@ -399,12 +403,19 @@ def main():
     hip.device_synchronize()
     t0 = time.perf_counter()
     evs[0].record()
-    for k in range(a.steps):
-        eng.run(a.interval)  # one kernel launch on the null stream
-        evs[k + 1].record()
+    if a.one_call:
+        eng.run(a.interval * a.steps)
+        evs[a.steps].record()
+    else:
+        for k in range(a.steps):
+            eng.run(a.interval)  # one kernel launch on the null stream
+            evs[k + 1].record()
     hip.device_synchronize()
     wall = time.perf_counter() - t0
-    launch_ms = [evs[k].elapsed_ms(evs[k + 1]) for k in range(a.steps)]
+    if a.one_call:
+        launch_ms = [evs[0].elapsed_ms(evs[a.steps]) / a.steps] * a.steps
+    else:
+        launch_ms = [evs[k].elapsed_ms(evs[k + 1]) for k in range(a.steps)]
     gpu_ms = sum(launch_ms)
     task_gens = a.tasks * a.interval * a.steps
     value = task_gens / (gpu_ms / 1e3)
@@ -452,7 +463,9 @@ def main():
                      "avg_launch_ms": gpu_ms / a.steps},
         "cpu_baseline": cpu,
         "time_to_1e-6": study,
-        "launch": ("epochs replayed from a captured hipGraph" if a.schedule == "auto" else "epochs launched directly"),
+        "launch": {"auto": "epochs replayed from a captured hipGraph", "epochs": "epochs launched directly",
+                   "persistent": "persistent schedule, epochs ordered on the device"}[a.schedule]
+                  + (f"; the {a.steps} steps as one run call" if a.one_call else ""),
         "wall_s": wall,
         "mean_best_fitness": float(best.mean()),
     }

@@ -97,8 +97,15 @@ enum { EMTF_FORMULAS_TEXTBOOK = 0, EMTF_FORMULAS_REFERENCE_DE = 1, EMTF_FORMULAS
  * when the epoch length changes): epochs of one whole transfer interval (or of any length
  * without transfer) whose end carries no trace point or polish; the kernels read their
  * first generation from a device counter that the graph advances.  Same results either way.
- * PERSISTENT (one launch per call, the mailbox ordered by device-side completion stamps) is
- * reserved and returns EMTF_E_UNSUPPORTED at create in this build. */
+ * PERSISTENT: one launch per call (per stretch between trace points, at most 1024 epochs
+ * each) of one workgroup per (epoch, task) item; each workgroup takes the next item, in the
+ * order the workgroups start, from a device counter, and waits, on the device, for its
+ * task's and its donor's previous epoch and, before it publishes, for the previous epoch's
+ * mailbox readers; every wait is bounded (10 s), and one that runs out makes emtf_run return
+ * EMTF_E_DEVICE.  emtf_run then waits for the stream.
+ * Only for handles whose tasks all fall in one LDS-kernel bucket (one row geometry, the
+ * population in LDS), without polish or online admission (else EMTF_E_UNSUPPORTED at
+ * create); the autotuner and the graph are off.  Same results as the other schedules. */
 enum { EMTF_SCHEDULE_AUTO = 0, EMTF_SCHEDULE_EPOCHS = 1, EMTF_SCHEDULE_PERSISTENT = 2 };
 
 /* When tasks start.  AT_CREATE: every task starts at emtf_create.  ONLINE: none does;

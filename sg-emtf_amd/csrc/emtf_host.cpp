@@ -60,6 +60,17 @@ EMTF_DECL_OBJ(2) EMTF_DECL_OBJ(4) EMTF_DECL_OBJ(7) EMTF_DECL_OBJ(8) EMTF_DECL_OB
 #define EMTF_DECL_OBJW(nt) extern "C" int emtf_launch_objective_nt##nt##_fwd(const emtf::ObjArgs* a, hipStream_t s);
 EMTF_DECL_OBJW(5) EMTF_DECL_OBJW(6) EMTF_DECL_OBJW(7) EMTF_DECL_OBJW(8)
 
+// the persistent schedule's kernel (LDS kernel builds) and its occupancy query
+#define EMTF_DECL_PERSIST(kb, al)                                                                           \
+    extern "C" int emtf_launch_persist_kb##kb##_a##al(const emtf::EpochArgs* args, const emtf::PersistArgs* pa, \
+                                                      const int* order, int blocks, int lds_bytes,         \
+                                                      hipStream_t stream);
+#define EMTF_DECL_PPAIR(kb) EMTF_DECL_PERSIST(kb, 0) EMTF_DECL_PERSIST(kb, 2)
+EMTF_DECL_PPAIR(2) EMTF_DECL_PPAIR(4) EMTF_DECL_PPAIR(7) EMTF_DECL_PPAIR(8) EMTF_DECL_PPAIR(12) EMTF_DECL_PPAIR(13)
+EMTF_DECL_PPAIR(16)
+typedef int (*persist_fn)(const emtf::EpochArgs*, const emtf::PersistArgs*, const int*, int, int, hipStream_t);
+extern "C" int emtf_launch_fill_i32(int32_t* p, int n, int32_t value, hipStream_t stream);
+
 typedef int (*launch_fn)(const emtf::EpochArgs*, const int*, int, int, hipStream_t);
 extern "C" int emtf_launch_trace(const emtf::TaskDev* tasks, int n_tasks, const float* fit, float* out_best,
                                  float* out_worst, hipStream_t stream);
@@ -131,6 +142,22 @@ static launch_fn pick_launcher(int kb, bool ga) {
 #undef EMTF_CASE
     return nullptr;
 }
+
+static persist_fn pick_persist(int kb, bool ga) {
+#define EMTF_CASE(k) \
+    case k: return ga ? emtf_launch_persist_kb##k##_a2 : emtf_launch_persist_kb##k##_a0;
+    switch (kb) {
+        EMTF_CASE(2) EMTF_CASE(4) EMTF_CASE(7) EMTF_CASE(8) EMTF_CASE(12) EMTF_CASE(13) EMTF_CASE(16)
+    }
+#undef EMTF_CASE
+    return nullptr;
+}
+
+// Epochs one persistent launch holds (its per-epoch read counters), and the bound of one
+// device-side wait (10 s of the 100 MHz wall clock): an item waits at most for another
+// item's epoch, far below it.
+constexpr int kPersistEpochs = 1024;
+constexpr int64_t kPersistWaitTicks = 1000000000LL;
 
 typedef int (*polish_lds_fn)(int, int, int);
 
@@ -247,6 +274,10 @@ struct emtf_handle {
     bool online = false;
     std::vector<int> ends;       // clock values at which admitted tasks finish (sorted)
     int32_t* d_donors = nullptr; // [T] donor list of a transfer boundary, then its count
+    // persistent schedule: its launcher and the device counters (error flag, item counter,
+    // T stamps, kPersistEpochs read counts)
+    persist_fn persist = nullptr;
+    int32_t* d_persist = nullptr;
 };
 
 static void free_handle(emtf_handle* h) {
@@ -266,6 +297,7 @@ static void free_handle(emtf_handle* h) {
     (void)hipFree(h->d_elite);
     (void)hipFree(h->d_gen);
     (void)hipFree(h->d_donors);
+    (void)hipFree(h->d_persist);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
@@ -807,7 +839,21 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     // its largest dimension, for a wide bucket, its 512-thread build; [2] the wide kernel
     // with Q in LDS where it fits (timed against [0] by the autotuner; not with polish,
     // whose kernel needs the LDS rows)
-    const bool tune = prm->tune != 1 && !want_polish && !online;
+    // the persistent schedule runs one LDS-kernel launch unit, without polish or admission
+    const bool persist = prm->schedule == EMTF_SCHEDULE_PERSISTENT;
+    if (persist) {
+        const char* why = nullptr;
+        if (want_polish) why = "persistent schedule: gradient polish needs the epoch schedule";
+        else if (online) why = "persistent schedule: online admission needs the epoch schedule";
+        else if (h->buckets.size() != 1 || h->buckets[0].wide)
+            why = "persistent schedule: every task must fall in one LDS-kernel bucket (one row geometry, "
+                  "the population in LDS)";
+        if (why) {
+            free_handle(h);
+            return fail(EMTF_E_UNSUPPORTED, why);
+        }
+    }
+    const bool tune = prm->tune != 1 && !want_polish && !online && !persist;
     std::vector<int> alt_nt(h->buckets.size(), -1);
     for (size_t k = 0; k < h->buckets.size(); ++k) {
         Bucket& b = h->buckets[k];
@@ -995,11 +1041,17 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     }
     h->mailbox_stride = nM > 0 ? nM : 1;
 
-    // The persistent schedule (one launch per run, mailbox ordered by device-side
-    // completion stamps) is switched off: its first GPU test did not complete.
-    if (prm->schedule == EMTF_SCHEDULE_PERSISTENT) {
-        free_handle(h);
-        return fail(EMTF_E_UNSUPPORTED, "persistent schedule is not available in this build");
+    if (persist) {
+        h->persist = pick_persist(h->buckets[0].geo, ga);
+        if (!h->persist) {
+            free_handle(h);
+            return fail(EMTF_E_UNSUPPORTED, "persistent schedule: no kernel built for this row geometry");
+        }
+        const hipError_t ep = hipMalloc((void**)&h->d_persist, sizeof(int32_t) * (2 + (size_t)n_tasks + kPersistEpochs));
+        if (ep != hipSuccess) {
+            free_handle(h);
+            return fail(EMTF_E_NOMEM, std::string("persistent schedule counters: ") + hipGetErrorString(ep));
+        }
     }
 
     {
@@ -1096,12 +1148,81 @@ int emtf_create(const emtf_task_desc* tasks, int32_t n_tasks, const emtf_params*
     return EMTF_OK;
 }
 
+// The persistent schedule: one launch per stretch of the run between trace points (at
+// most kPersistEpochs epochs each), then the trace kernel where one falls.  Synchronous: the
+// launches' error flag is read back at the end (a device-side wait that ran out).
+static int run_persistent(emtf_handle* h, int generations, hipStream_t s) {
+    const int TI = h->p.transfer_interval;
+    const bool xfer = TI > 0 && h->T > 1 && h->p.transfer_count > 0;
+    const int g_end = h->gen + generations;
+    int32_t* err = h->d_persist;  // then the item counter, T stamps, the read counts
+    int32_t* next = err + 1;
+    const Bucket& b = h->buckets[0];
+    emtf::EpochArgs a{};
+    a.tasks = h->d_tasks;
+    a.n_tasks = h->T;
+    a.X = h->d_X;
+    a.fit = h->d_fit;
+    a.Q = h->d_Q;
+    a.shift = h->d_shift;
+    a.tab = h->d_tab;
+    a.mailbox = h->d_mailbox;
+    a.mailbox_stride = h->mailbox_stride;
+    a.scratch = h->d_scratch;
+    a.Qw = h->d_qw;
+    a.p = h->p;
+    int e = emtf_launch_fill_i32(err, 1, 0, s);
+    if (e != 0) return fail(EMTF_E_DEVICE, std::string("persistent counters: ") + hipGetErrorString((hipError_t)e));
+    while (h->gen < g_end) {
+        int g1 = g_end;
+        const int TR = h->trace_points < h->trace_max ? h->trace_interval : 0;
+        if (TR > 0) {
+            const int nb = h->trace_g0 + ((h->gen - h->trace_g0) / TR + 1) * TR;
+            if (nb < g1) g1 = nb;
+        }
+        const bool trace_here = TR > 0 && (g1 - h->trace_g0) % TR == 0;
+        int n_ep = 1;
+        if (xfer) {
+            const int lim = (h->gen / TI + kPersistEpochs) * TI;
+            if (lim < g1) g1 = lim;
+            n_ep = (g1 - 1) / TI - h->gen / TI + 1;  // cut at the boundaries inside (gen, g1)
+        }
+        e = emtf_launch_fill_i32(next, 1 + h->T + n_ep, 0, s);
+        if (e != 0) return fail(EMTF_E_DEVICE, std::string("persistent counters: ") + hipGetErrorString((hipError_t)e));
+        emtf::PersistArgs pa{};
+        pa.G0 = h->gen;
+        pa.G1 = g1;
+        pa.n_epochs = n_ep;
+        pa.next = next;
+        pa.err = err;
+        pa.stamp = next + 1;
+        pa.reads = next + 1 + h->T;
+        pa.wait_ticks = kPersistWaitTicks;
+        const int blocks = n_ep * h->T;  // one workgroup per item
+        const int rc = h->persist(&a, &pa, h->d_order + b.offset, blocks, b.v[0].lds_bytes, s);
+        if (rc != 0) return fail(EMTF_E_DEVICE, std::string("persistent kernel launch: ") + hipGetErrorString((hipError_t)rc));
+        h->gen = g1;
+        h->gen_valid = false;
+        if (trace_here) {
+            e = emtf_launch_trace(h->d_tasks, h->T, h->d_fit, h->d_trace + (int64_t)h->trace_points * h->T, nullptr, s);
+            if (e != 0) return fail(EMTF_E_DEVICE, std::string("trace: ") + hipGetErrorString((hipError_t)e));
+            ++h->trace_points;
+        }
+    }
+    int32_t flag = 0;
+    HIP_OR_FAIL(hipMemcpyAsync(&flag, err, sizeof flag, hipMemcpyDeviceToHost, s));
+    HIP_OR_FAIL(hipStreamSynchronize(s));
+    if (flag) return fail(EMTF_E_DEVICE, "persistent schedule: a device-side wait ran out");
+    return EMTF_OK;
+}
+
 int emtf_run(emtf_handle* h, int32_t generations, void* stream) {
     g_err.clear();
     if (!h) return fail(EMTF_E_INVALID, "handle is NULL");
     if (generations < 0) return fail(EMTF_E_INVALID, "generations < 0");
     hipStream_t s = (hipStream_t)stream;
     if (const int rc = use_stream(h, s); rc != EMTF_OK) return rc;
+    if (h->persist) return run_persistent(h, generations, s);
     const int TI = h->p.transfer_interval;
     const bool xfer = TI > 0 && h->T > 1 && h->p.transfer_count > 0;
     const int g_end = h->gen + generations;

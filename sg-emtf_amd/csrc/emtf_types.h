@@ -73,6 +73,22 @@ struct EpochArgs {
     const int32_t* n_donors;
 };
 
+// The persistent schedule: one launch runs generations G0+1 .. G1 of every task as a
+// sequence of epochs cut at the transfer boundaries.  Items (epoch e, task order[i]),
+// e-major, are taken from a device counter; an item waits (thread 0, device-scope acquire
+// loads, a bounded wait) for its task's previous epoch and, at a boundary, for its donor's
+// previous epoch (whose end published the elites it reads); an item that publishes first
+// waits until every item of the previous epoch has read the mailbox buffer it overwrites.
+struct PersistArgs {
+    int32_t G0, G1;        // generations G0+1 .. G1
+    int32_t n_epochs;      // epochs in this launch
+    int32_t* next;         // item counter (starts at 0)
+    int32_t* err;          // set to 1 when a wait ran out (starts at 0)
+    int32_t* stamp;        // [n_tasks] epochs of this launch each task has finished (start 0)
+    int32_t* reads;        // [n_epochs] items of each epoch past their mailbox read (start 0)
+    int64_t wait_ticks;    // bound of one wait in wall-clock ticks (100 MHz)
+};
+
 // Stand-alone objective / adjoint evaluation of n points (device pointers).
 struct ObjArgs {
     int32_t func, dim, n, has_rot;

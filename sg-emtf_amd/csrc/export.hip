@@ -143,7 +143,20 @@ __global__ void gen_counter_kernel(int32_t* gen, int32_t value, int32_t add) {
     if (threadIdx.x == 0) gen[0] = add ? gen[0] + value : value;
 }
 
+// n int32 values set to v (the persistent schedule's counters; a kernel, not a memset, so
+// the call can sit in a captured graph).
+__global__ void fill_i32_kernel(int32_t* p, int n, int32_t v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 }  // namespace emtf
+
+extern "C" int emtf_launch_fill_i32(int32_t* p, int n, int32_t value, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const int blocks = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+    hipLaunchKernelGGL(emtf::fill_i32_kernel, dim3(blocks), dim3(256), 0, stream, p, n, value);
+    return (int)hipGetLastError();
+}
 
 extern "C" int emtf_launch_gen_counter(int32_t* gen, int32_t value, int32_t add, hipStream_t stream) {
     hipLaunchKernelGGL(emtf::gen_counter_kernel, dim3(1), dim3(64), 0, stream, gen, value, add);

@@ -102,11 +102,13 @@ __device__ __forceinline__ bool runs_epoch(const EpochArgs& a, const TaskDev& td
 
 // Whether the task receives elites at the start of this epoch: a transfer boundary that
 // is not its own start, with another donor to draw.
-__device__ __forceinline__ bool transfer_now(const EpochArgs& a, const TaskDev& td) {
+// (g0: the epoch's first generation, a.g0 but for the persistent schedule's items)
+__device__ __forceinline__ bool transfer_now(const EpochArgs& a, const TaskDev& td, int g0) {
     const int TI = a.p.transfer_interval;
-    if (a.init || TI <= 0 || a.p.transfer_count <= 0 || a.g0 % TI != 0 || a.g0 <= td.gen_base) return false;
+    if (a.init || TI <= 0 || a.p.transfer_count <= 0 || g0 % TI != 0 || g0 <= td.gen_base) return false;
     return a.donors ? *a.n_donors > 1 : a.n_tasks > 1;
 }
+__device__ __forceinline__ bool transfer_now(const EpochArgs& a, const TaskDev& td) { return transfer_now(a, td, a.g0); }
 
 // The donor of task `task` at a transfer event: the k-th of the other donors in ascending
 // id order, k = w mod (n - 1) -- every task of the handle, or the online donor list.
@@ -946,17 +948,17 @@ __device__ __forceinline__ void island_migration(const EpochArgs& a, const Geo& 
 // Inter-task transfer at the start of an epoch (reference knowledgeTransfer).
 template <int KB, int FUNC>
 __device__ __forceinline__ void transfer_in(const EpochArgs& a, const TaskDev& td, const Geo& g, const Lds& L,
-                                            const Rng& rng, int tid) {
+                                            const Rng& rng, int tid, int eg0) {
     using S = Shape<KB>;
-    // draws at the task's own generation g0, the mailbox buffer by the handle's clock
-    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = a.g0 - td.gen_base;
+    // draws at the task's own generation g0, the mailbox buffer by the handle's clock eg0
+    const int k = a.p.transfer_count, TI = a.p.transfer_interval, g0 = eg0 - td.gen_base;
     const int wave = tid >> 6, lane = tid & 63, col = lane & 31, h = lane >> 5;
     rank_islands(L, g.pop, g.pop, tid);
     __syncthreads();
     const uint32_t wd = rng.word((uint32_t)g0, P_MIG, 0u, 0u, 0u);
     const int dn = pick_donor(a, (int)rng.task, wd);
     const TaskDev dd = a.tasks[dn];
-    const float* mb = a.mailbox + (int64_t)((a.g0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
+    const float* mb = a.mailbox + (int64_t)((eg0 / TI) & 1) * a.mailbox_stride + dd.mb_off;
     const bool same_box = (dd.lo == td.lo && dd.hi == td.hi);
     const float rng_s = td.hi - td.lo, rng_d = dd.hi - dd.lo;
     const int ntc = (k + 31) / 32;
@@ -1509,8 +1511,30 @@ __global__ __launch_bounds__(kThreads) void polish_kernel(EpochArgs a, const int
     }
 }
 
-template <int KB, int FUNC, int ALGO>
-__device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td) {
+// Thread 0's bounded wait for *p >= v under the persistent schedule (device-scope acquire
+// loads; false once the handle's error flag is set or the wait runs out, which sets it).
+__device__ __noinline__ bool persist_wait(const int32_t* p, int32_t v, const PersistArgs& pa) {
+    const uint64_t t0 = wall_clock64();
+    for (;;) {
+        if (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= v) return true;
+        if (__hip_atomic_load(pa.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if ((int64_t)(wall_clock64() - t0) > pa.wait_ticks) {
+            __hip_atomic_store(pa.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(8);
+    }
+}
+
+// PERSIST: the epoch is item (pe, task) of a persistent launch -- it counts itself in
+// pa->reads[pe] once past its mailbox read, and waits for every item of epoch pe - 1 to
+// be past theirs before it overwrites their mailbox buffer.
+template <int KB, int FUNC, int ALGO, bool PERSIST = false>
+__device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, const int task, const TaskDev& td,
+                                           const PersistArgs* pa = nullptr, int pe = 0, int pg0 = 0, int pg1 = 0,
+                                           int ppub = 0) {
+    // the epoch's generations and publish flag: the launch's, or the persistent item's
+    const int eg0 = PERSIST ? pg0 : a.g0, eg1 = PERSIST ? pg1 : a.g1, epub = PERSIST ? ppub : a.publish;
     using S = Shape<KB>;
     constexpr int RS = S::RS, HS = S::HS, NT = S::NT;
     const int tid = threadIdx.x;
@@ -1566,10 +1590,14 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
     __syncthreads();
 
     const int TI = a.p.transfer_interval;
-    if (transfer_now(a, td)) transfer_in<KB, FUNC>(a, td, g, L, rng, tid);
+    if (transfer_now(a, td, eg0)) transfer_in<KB, FUNC>(a, td, g, L, rng, tid, eg0);
+    if constexpr (PERSIST) {
+        __syncthreads();
+        if (tid == 0) __hip_atomic_fetch_add(pa->reads + pe, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
 
     const bool islands = a.p.n_islands > 1;
-    for (int gen = a.g0 - td.gen_base + 1; gen <= a.g1 - td.gen_base; ++gen) {
+    for (int gen = eg0 - td.gen_base + 1; gen <= eg1 - td.gen_base; ++gen) {
         if (islands && a.p.exchange_interval > 0 && gen % a.p.exchange_interval == 0)
             island_exchange<KB>(a, g, L, a.scratch + td.scr_off, rng, gen, tid);
         generation_step<KB, FUNC, ALGO>(a, td, g, L, rng, gen, tid);
@@ -1583,12 +1611,18 @@ __device__ __forceinline__ void epoch_body(const EpochArgs& a, float* smem, cons
         Xg[e] = L.X[i * RS + (d & 1) * HS + (d >> 1)];
     }
     for (int i = tid; i < g.pop; i += kThreads) a.fit[td.fit_off + i] = L.fit[i];
-    if (a.publish) {
+    if (epub) {
         __syncthreads();
         rank_islands(L, g.pop, g.pop, tid);
+        if constexpr (PERSIST) {
+            __shared__ int s_pub;
+            if (tid == 0) s_pub = (pe == 0 || persist_wait(pa->reads + pe - 1, a.n_tasks, *pa)) ? 1 : 0;
+            __syncthreads();
+            if (!s_pub) return;  // the error flag is set: the launch drains
+        }
         __syncthreads();
         const int k = a.p.transfer_count;
-        float* mb = a.mailbox + (int64_t)((a.g1 / TI) & 1) * a.mailbox_stride + td.mb_off;
+        float* mb = a.mailbox + (int64_t)((eg1 / TI) & 1) * a.mailbox_stride + td.mb_off;
         for (int e = tid; e < k * g.D; e += kThreads) {
             const int r = e / g.D, d = e - r * g.D;
             mb[e] = L.X[L.ord[r] * RS + (d & 1) * HS + (d >> 1)];
@@ -1635,6 +1669,79 @@ __global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_epoch_kernel(E
     }
 }
 
+
+// The persistent schedule: generations pa.G0+1 .. pa.G1 of every task in one launch of
+// n_epochs x n_tasks workgroups.  Each workgroup takes the next item (epoch e, task
+// order[i]), e-major, from pa.next in the order the workgroups start (not by blockIdx), so
+// every item it may wait on was taken earlier by a workgroup that is running; it waits for
+// its task's epoch e - 1 and, at a transfer boundary, for its donor's (the one transfer_in
+// draws), runs the epoch (PERSIST hooks: the mailbox read count and the wait before
+// publishing) and releases pa.stamp[task] = e + 1.  No wait targets a later item, so the
+// launch finishes whatever the number of resident workgroups; every wait is bounded.
+template <int KB, int ALGO>
+__global__ __launch_bounds__(kThreads) EMTF_OCCUPANCY void island_persistent_kernel(EpochArgs a, PersistArgs pa,
+                                                                                   const int* __restrict__ order) {
+    extern __shared__ float smem[];
+    __shared__ int s_item, s_ok;
+    const int tid = threadIdx.x;
+    const int TI = a.p.transfer_interval;
+    const bool xfer = TI > 0 && a.n_tasks > 1 && a.p.transfer_count > 0;
+    const int n_items = pa.n_epochs * a.n_tasks;
+    {
+        if (tid == 0) s_item = __hip_atomic_fetch_add(pa.next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= n_items) return;
+        const int e = item / a.n_tasks;
+        const int task = order[item - e * a.n_tasks];
+        int g0, g1, pub;
+        if (xfer) {
+            const int b0 = pa.G0 / TI;  // epochs end at the transfer boundaries
+            g0 = e == 0 ? pa.G0 : (b0 + e) * TI;
+            g1 = min((b0 + e + 1) * TI, pa.G1);
+            pub = g1 % TI == 0 ? 1 : 0;
+        } else {
+            g0 = pa.G0;
+            g1 = pa.G1;
+            pub = 0;
+        }
+        const TaskDev td = a.tasks[task];
+        if (tid == 0) {
+            bool ok = persist_wait(pa.stamp + task, e, pa);
+            if (ok && transfer_now(a, td, g0)) {
+                const Rng rng{a.p.seed_lo, a.p.seed_hi, (uint32_t)task};
+                const uint32_t wd = rng.word((uint32_t)(g0 - td.gen_base), P_MIG, 0u, 0u, 0u);
+                ok = persist_wait(pa.stamp + pick_donor(a, task, wd), e, pa);
+            }
+            s_ok = ok ? 1 : 0;
+        }
+        __syncthreads();
+        if (!s_ok) return;  // the error flag is set: the launch drains
+        // (the waits' acquire loads and the stamp's release store, with the workgroup
+        // barriers around them, order the rows, fitness and mailbox across workgroups)
+        switch (td.func) {
+        case F_SPHERE: epoch_body<KB, F_SPHERE, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_ROSENBROCK: epoch_body<KB, F_ROSENBROCK, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_ACKLEY: epoch_body<KB, F_ACKLEY, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_RASTRIGIN: epoch_body<KB, F_RASTRIGIN, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_GRIEWANK: epoch_body<KB, F_GRIEWANK, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_WEIERSTRASS: epoch_body<KB, F_WEIERSTRASS, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_SCHWEFEL: epoch_body<KB, F_SCHWEFEL, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_ELLIPTIC: epoch_body<KB, F_ELLIPTIC, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_REF_ACKLEY: epoch_body<KB, F_REF_ACKLEY, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_REF_RASTRIGIN: epoch_body<KB, F_REF_RASTRIGIN, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_REF_GRIEWANK: case F_REF_GRIEWANK_GA:
+            epoch_body<KB, F_REF_GRIEWANK, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub);
+            break;
+        case F_REF_WEIERSTRASS: epoch_body<KB, F_REF_WEIERSTRASS, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        case F_REF_SCHWEFEL: epoch_body<KB, F_SCHWEFEL, ALGO, true>(a, smem, task, td, &pa, e, g0, g1, pub); break;
+        default: break;
+        }
+        __syncthreads();
+        if (tid == 0 && __hip_atomic_load(pa.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            __hip_atomic_store(pa.stamp + task, e + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
 
 template <int KB, int FUNC, bool ADJ>
 __global__ __launch_bounds__(2 * ObjShape<KB>::PB) void objective_kernel(ObjArgs a) {
@@ -2280,6 +2387,32 @@ extern "C" int EMTF_NAME(emtf_launch_epoch_kb, EMTF_KB, a, EMTF_ALGO)(const emtf
     hipLaunchKernelGGL(k, dim3(n), dim3(emtf::kThreads), lds_bytes, stream, *args, order);
     return (int)hipGetLastError();
 }
+
+// emtf_launch_persist_kb<KB>_a<ALGO>: the persistent schedule's kernel over `blocks`
+// workgroups (one per item).
+// the kernel's dynamic LDS allowance: 160 KB less its own static LDS (its item slot)
+static hipError_t persist_lds_setup() {
+    static bool done = false;
+    if (done) return hipSuccess;
+    const void* k = reinterpret_cast<const void*>(&emtf::island_persistent_kernel<EMTF_KB, EMTF_ALGO>);
+    hipFuncAttributes fa{};
+    hipError_t e = hipFuncGetAttributes(&fa, k);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - (int)fa.sharedSizeBytes);
+    if (e == hipSuccess) done = true;
+    return e;
+}
+
+extern "C" int EMTF_NAME(emtf_launch_persist_kb, EMTF_KB, a, EMTF_ALGO)(const emtf::EpochArgs* args,
+                                                                       const emtf::PersistArgs* pa, const int* order,
+                                                                       int blocks, int lds_bytes, hipStream_t stream) {
+    const hipError_t e = persist_lds_setup();
+    if (e != hipSuccess) return (int)e;
+    auto k = &emtf::island_persistent_kernel<EMTF_KB, EMTF_ALGO>;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(emtf::kThreads), lds_bytes, stream, *args, *pa, order);
+    return (int)hipGetLastError();
+}
+
 
 #if EMTF_KB == 2 && EMTF_ALGO == 0
 extern "C" int emtf_lds_floats_kb(int pop, int kb) { return emtf::lds_floats_kb(pop, kb); }

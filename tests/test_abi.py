@@ -139,12 +139,22 @@ def test_null_arguments():
     assert L.emtf_destroy(None) == 0
 
 
-def test_persistent_schedule_rejected():
+def test_persistent_schedule_limits():
+    """The persistent schedule runs one LDS-kernel bucket without polish or admission; the
+    other handles are refused at create, before any device work."""
     import emtf
     from emtf import tasks as tg
     tasks = tg.benchmark_tasks(2, 10, 32, rotate=False, seed=2)
-    with pytest.raises(emtf.EmtfError, match="persistent"):
-        emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, transfer_interval=5, transfer_count=2)
+    with pytest.raises(emtf.EmtfError, match="persistent schedule: gradient polish"):
+        emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, polish_interval=5, polish_steps=1, polish_topk=1)
+    with pytest.raises(emtf.EmtfError, match="persistent schedule: online admission"):
+        emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, admission=emtf.ADMIT_ONLINE)
+    two = tg.benchmark_tasks(2, 0, 32, rotate=False, seed=2, dims=[10, 100], pops=[32, 32])
+    with pytest.raises(emtf.EmtfError, match="persistent schedule: every task must fall in one"):
+        emtf.Engine(two, schedule=emtf.SCHEDULE_PERSISTENT, transfer_interval=5, transfer_count=2)
+    wide = tg.benchmark_tasks(2, 150, 32, rotate=False, seed=2)
+    with pytest.raises(emtf.EmtfError, match="persistent schedule: every task must fall in one"):
+        emtf.Engine(wide, schedule=emtf.SCHEDULE_PERSISTENT)
     with pytest.raises(emtf.EmtfError, match="schedule"):
         emtf.Engine(tasks, schedule=7)
 

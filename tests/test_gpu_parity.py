@@ -136,17 +136,16 @@ def test_uneven_batch_buckets(oracle_mod, algo):
         _compare(gpu, orc, f"uneven algo {algo} gen {orc.generation}")
 
 
-# ---- persistent schedule (one launch per run, device-side mailbox ordering)
-PERSISTENT_OFF = ("persistent schedule switched off: its first GPU run did not complete within the test timeout "
-                  "and no cause has been shown; the library rejects schedule=PERSISTENT")
+# ---- persistent schedule (one launch per run, device-side mailbox ordering; its ordering
+# protocol is modelled on the CPU in test_persist_model.py)
 
 
-@pytest.mark.skip(reason=PERSISTENT_OFF)
 @pytest.mark.parametrize("algo", [emtf.DE_RAND_1_BIN, emtf.GA_SBX_PM])
 def test_persistent_schedule_bit_exact(oracle_mod, algo):
     tasks = tg.benchmark_tasks(24, 40, 64, rotate=True, seed=12)
     params = dict(seed=4, algo=algo, n_islands=2, exchange_interval=3, migrate_interval=4, migrate_count=2,
                   transfer_interval=5, transfer_count=3, transfer_cr=0.2)
+    print(f"case: persistent schedule, 24 tasks D=40 pop 64, algo {algo}", flush=True)
     per = emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, **params)
     epo = emtf.Engine(tasks, schedule=emtf.SCHEDULE_EPOCHS, **params)
     orc = oracle_mod.Engine(tasks, **params)
@@ -158,16 +157,33 @@ def test_persistent_schedule_bit_exact(oracle_mod, algo):
         _compare(epo, orc, f"epochs gen {orc.generation}")
 
 
-@pytest.mark.skip(reason=PERSISTENT_OFF)
 def test_persistent_many_epochs_per_call(oracle_mod):
     """More epochs than one persistent launch holds: the call is split into chunks."""
     tasks = tg.benchmark_tasks(6, 10, 32, rotate=False, seed=1)
     params = dict(seed=9, transfer_interval=1, transfer_count=2)
+    print("case: persistent schedule, 6 tasks, 1030 one-generation epochs", flush=True)
     per = emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, **params)
     orc = oracle_mod.Engine(tasks, **params)
     per.run(1030)
     orc.run(1030)
     _compare(per, orc, "1030 one-generation epochs")
+
+
+def test_persistent_without_transfer_and_with_trace(oracle_mod):
+    """No transfer (one epoch per launch), and launches cut at the convergence-trace points:
+    the trace and the populations equal the epoch schedule's."""
+    tasks = tg.benchmark_tasks(40, 20, 64, rotate=True, seed=5)
+    print("case: persistent schedule without transfer / with trace", flush=True)
+    for params in (dict(seed=3), dict(seed=3, transfer_interval=4, transfer_count=2)):
+        per = emtf.Engine(tasks, schedule=emtf.SCHEDULE_PERSISTENT, **params)
+        epo = emtf.Engine(tasks, schedule=emtf.SCHEDULE_EPOCHS, **params)
+        per.trace_start(3, 10)
+        epo.trace_start(3, 10)
+        for step in (7, 11):
+            per.run(step)
+            epo.run(step)
+        _compare(per, epo, f"persistent vs epochs {params}")
+        assert np.array_equal(per.trace().view(np.uint32), epo.trace().view(np.uint32))
 
 
 @pytest.mark.parametrize("tune", [0, 1, 2, 3])
