@@ -27,8 +27,38 @@ def fit(name, f, a, b, n, scale=None):
     return c32
 
 
+def fit_rel_minimax(name, f, a, b, deg, iters=60):
+    """Least squares on a 20001-point grid, reweighted toward the minimax of the relative
+    error (Lawson's iteration), lowest degree first; prints the binary32 Horner error."""
+    xs = np.linspace(a, b, 20001)
+    ys = f(xs)
+    V = np.vander(xs, deg + 1, increasing=True)
+    w = np.ones_like(xs)
+    for _ in range(iters):
+        W = w / np.abs(ys)
+        c, *_ = np.linalg.lstsq(V * W[:, None], ys * W, rcond=None)
+        err = (V @ c - ys) / ys
+        w = w * (np.abs(err) + 1e-30) ** 0.5
+        w /= w.mean()
+    c32 = c.astype(np.float32)
+    x32 = xs.astype(np.float32)
+    p = np.full_like(x32, c32[-1])
+    for k in range(deg - 1, -1, -1):
+        p = (p * x32 + c32[k]).astype(np.float32)
+    rel = np.abs((p.astype(np.float64) - ys) / ys).max()
+    print(f"{name}: degree {deg} on [{a}, {b}], max rel err (binary32 Horner) {rel:.3e}")
+    print("  " + ", ".join(f"{float(v)!r}f" for v in c32))
+    return c32
+
+
 def main():
     pi = mp.pi
+    # sin^2(pi t) = x * WQ(x), x = t^2, t in [0, 1/2] (the Weierstrass terms, lowest degree first)
+    def wq(x):
+        t = np.sqrt(x)
+        return np.where(x > 0, np.sin(np.pi * t) ** 2 / np.where(x > 0, x, 1.0), np.pi ** 2)
+    for d in (5, 4):
+        fit_rel_minimax(f"WQ{d}", wq, 0.0, 0.25, d)
     # sin(pi*a) = a * SP(x), x = a^2, a in [0, 1/2]
     fit("SP", lambda x: (mp.sin(pi * mp.sqrt(x)) / mp.sqrt(x)) if x > 0 else pi, 0.0, 0.25, 5)
     # sin(2*pi*b) = b * SN(x), x = b^2, b in [0, 1/4]

@@ -188,11 +188,23 @@ __device__ __forceinline__ float pow_f(float x, float y) {
     return exp2_f(ly);
 }
 
+// sin^2(pi t) = x Q_d(x), x = t^2, t in [-1/2, 1/2]: Q_d of degree d fitted to
+// sin^2(pi sqrt(x)) / x on [0, 1/4] by least squares reweighted toward the minimax of the
+// relative error (Lawson's iteration; oracle/fit_coeffs.py); largest relative error of the
+// binary32 Horner form: d = 5: 1.9e-7, d = 4: 1.2e-6 (also at x -> 0, so a sum of terms
+// keeps its relative accuracy near the optimum).
+constexpr float kWQ5[6] = {9.869604110717773f, -32.469661712646484f, 42.726924896240234f, -30.098546981811523f,
+                           13.03709602355957f, -3.336198329925537f};
+constexpr float kWQ4[5] = {9.86959457397461f, -32.46799087524414f, 42.67863464355469f, -29.595659255981445f,
+                           10.860730171203613f};
+
 // Weierstrass term sum_{k=0}^{20} 2^(1-k) sin^2(pi 3^k z).  For 2^-32 <= |z| < 2^23
 // the fractional part of 3^k |z| is carried exactly as a 0.64 fixed-point word
 // (3x mod 1 is one 64-bit shift-add); each term reads its top 32 bits as a signed
 // word: t = q 2^-32 in [-1/2, 1/2) differs from the fraction by 0 or 1, so
-// sin^2(pi t) is the term, and sinpi_half is odd, so its square needs no |t|.
+// sin^2(pi t) is the term.  Term k is 2^(1-k) x Q_d(x), the weight folded into the
+// coefficients (exact: powers of two), with d = 5 for the two heaviest terms and d = 4
+// for the others: relative error 1.2e-6 at most per term and for the sum.
 __device__ __forceinline__ float weier_term(float z) {
     if (z != z) return z;
     const float a = __builtin_fabsf(z);
@@ -223,8 +235,19 @@ __device__ __forceinline__ float weier_term(float z) {
         float qf;
         asm("v_cvt_f32_i32 %0, %1" : "=v"(qf) : "v"((uint32_t)(y >> 32)));
         const float t = qf * 2.3283064365386963e-10f;
-        const float sp = sinpi_half(t);
-        acc = fmaf(pow2i(1 - k), sp * sp, acc);
+        const float x = t * t;
+        const float w = pow2i(1 - k);
+        float q;
+        if (k < 2) {
+            q = kWQ5[5] * w;
+#pragma unroll
+            for (int i = 4; i >= 0; --i) q = fmaf(q, x, kWQ5[i] * w);
+        } else {
+            q = kWQ4[4] * w;
+#pragma unroll
+            for (int i = 3; i >= 0; --i) q = fmaf(q, x, kWQ4[i] * w);
+        }
+        acc = fmaf(x, q, acc);
         // y <- 3 y mod 2^64 in one VALU op (the backend otherwise emits two 32-bit
         // multiplies and register moves per step)
         asm("v_lshl_add_u64 %0, %1, 1, %1" : "=v"(y) : "v"(y));
