@@ -1183,8 +1183,10 @@ static int run_persistent(emtf_handle* h, int generations, hipStream_t s) {
         const bool trace_here = TR > 0 && (g1 - h->trace_g0) % TR == 0;
         int n_ep = 1;
         if (xfer) {
-            const int lim = (h->gen / TI + kPersistEpochs) * TI;
-            if (lim < g1) g1 = lim;
+            // at most kPersistEpochs epochs, and items (epochs x tasks) within 2^30
+            const int max_ep = std::max(1, std::min(kPersistEpochs, (1 << 30) / h->T));
+            const int64_t lim = ((int64_t)(h->gen / TI) + max_ep) * TI;
+            if (lim < g1) g1 = (int)lim;
             n_ep = (g1 - 1) / TI - h->gen / TI + 1;  // cut at the boundaries inside (gen, g1)
         }
         e = emtf_launch_fill_i32(next, 1 + h->T + n_ep, 0, s);
