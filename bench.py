@@ -52,16 +52,22 @@ def parse():
                     help="generations of the CPU-baseline sample (0 = skip; default: plain 2000 (GA 400), uneven "
                          "1000 (GA 200), cvrp 2000 -- 10 to 20 s of CPU work on 16 threads)")
     ap.add_argument("--cpu-tasks", type=int, default=1024)
-    ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "cvrp", "polish", "online"],
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "uneven", "cvrp", "polish", "online", "convergence"],
                     help="plain: the rotated continuous workload (default); uneven: 4096 tasks with "
                          "D~U{10..200}, pop~U{32..256}; cvrp: the CVRP GA workload; "
                          "polish: time to mean error 1e-6, adjoint polish vs plain DE at D=100; "
-                         "online: 10000 plain tasks admitted in 100 batches at U[0,100) ms gaps")
+                         "online: 10000 plain tasks admitted in 100 batches at U[0,100) ms gaps; "
+                         "convergence: per-objective FP64 error statistics of the plain configuration "
+                         "(--tasks/--dim/--pop/--gens) over seeds 0..N-1")
     ap.add_argument("--batches", type=int, default=100, help="online workload: arrival batches")
     ap.add_argument("--algo", default="de", choices=["de", "ga"],
                     help="plain / uneven workloads: DE/rand/1/bin (default) or the reference's GA operators "
                          "(SBX eta_c 2 + polynomial mutation eta_m 5, pm 1)")
-    ap.add_argument("--seeds", type=int, default=10, help="polish workload: seeds 0..N-1")
+    ap.add_argument("--seeds", type=int, default=10, help="polish / convergence workloads: seeds 0..N-1")
+    ap.add_argument("--no-rotate", action="store_true",
+                    help="convergence workload: unrotated tasks (the CPU configuration: --tasks 8 --pop 64 "
+                         "--gens 1000 --interval 10 --elites 2 --no-rotate)")
     ap.add_argument("--no-tune", action="store_true", help="switch the kernel-variant / stream autotuner off")
     ap.add_argument("--gens", type=int, default=3000, help="polish workload: generations per run")
     ap.add_argument("--only-func", default=None, help="give every task this objective (per-objective timing)")
@@ -313,6 +319,54 @@ def bench_polish(a):
     print(json.dumps(out))
 
 
+def bench_convergence(a):
+    """Convergence statistics of the plain configuration over seeds 0..N-1 (default: 1024
+    rotated tasks, D=50, pop 128, DE/rand/1/bin F=0.5 CR=0.9, 4 elites every 20 gens, 2000
+    generations; --tasks 8 --pop 64 --gens 1000 --interval 10 --elites 2 --no-rotate gives the
+    CPU configuration: 8 unrotated tasks, 2 elites every 10 gens).  Error =
+    float64 f(best) - f* of every task after every 100 generations (Schwefel excluded: its
+    minimum lies outside the box).  Per objective, over the seeds: the final mean error
+    (mean and standard deviation), the fraction of tasks at 1e-6 and the GPU time until the
+    objective's mean error reaches 1e-6."""
+    import numpy as np
+    import emtf
+    from emtf import tasks as tg
+    gens = a.gens if a.gens != 3000 else 2000
+    chunk, target = 100, 1e-6
+    rows = {n: [] for n in NAMES if n != "schwefel"}
+    gpu_ms = []
+    for seed in range(a.seeds):
+        tasks = tg.benchmark_tasks(a.tasks, a.dim, a.pop, rotate=not a.no_rotate, seed=seed)
+        funcs = np.array([t["func"] for t in tasks])
+        p = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=a.interval, transfer_count=a.elites,
+                 seed=seed)
+        t_ms, errs = _converge(tasks, p, gens, chunk, target)
+        gpu_ms.append(float(t_ms[-1]))
+        for f, n in enumerate(NAMES):
+            if n == "schwefel" or not np.any(funcs == f):
+                continue
+            m = np.mean(errs[:, funcs == f], axis=1)
+            rows[n].append((float(m[-1]), float(np.mean(errs[-1, funcs == f] <= target)), _time_to(t_ms, m, target)))
+        print(f"seed {seed}: {gens} generations, {t_ms[-1]:.1f} ms GPU", file=sys.stderr, flush=True)
+    per = {}
+    for n, r in rows.items():
+        if not r:
+            continue
+        fin = np.array([x[0] for x in r])
+        tt = [x[2] for x in r if x[2] is not None]
+        per[n] = {"final_mean_error_mean": float(fin.mean()), "final_mean_error_std": float(fin.std()),
+                  "frac_tasks_at_1e-6": float(np.mean([x[1] for x in r])), "runs_reaching_1e-6": len(tt),
+                  "mean_ms_to_1e-6": float(np.mean(tt)) if tt else None}
+    out = {"metric": "per-objective FP64 error after the run, mean over seeds", "value": None, "unit": "error",
+           "n_gpus": 1, "steps": gens // chunk, "warmup": 0, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"plain: {a.tasks} {'unrotated' if a.no_rotate else 'rotated'} tasks, D={a.dim}, pop={a.pop}, DE/rand/1/bin F=0.5 CR=0.9, "
+                                  f"{a.elites} elites every {a.interval} gens",
+                      "generations": gens, "seeds": a.seeds, "target": target},
+           "gpu_ms_per_run": float(np.mean(gpu_ms)), "per_objective": per}
+    print(json.dumps(out))
+
+
 def bench_online(a):
     """Online admission (the reference's driver, SG-EMTF.cu:1842-1958): T tasks of the plain
     configuration split into batches at random cut points, batch i arriving U[0, 100) ms after
@@ -373,6 +427,9 @@ def main():
         return
     if a.workload == "online":
         bench_online(a)
+        return
+    if a.workload == "convergence":
+        bench_convergence(a)
         return
     import emtf
     from emtf import hip
