@@ -53,12 +53,12 @@ def fit_rel_minimax(name, f, a, b, deg, iters=60):
 
 def main():
     pi = mp.pi
-    # sin^2(pi t) = x * WQ(x), x = t^2, t in [0, 1/2] (the Weierstrass terms, lowest degree first)
+    # sin^2(pi t) = x * WQ(x), x = t^2, t in [0, 1/2] (the Weierstrass even terms, lowest degree
+    # first; the odd terms follow by the triple-angle identity)
     def wq(x):
         t = np.sqrt(x)
         return np.where(x > 0, np.sin(np.pi * t) ** 2 / np.where(x > 0, x, 1.0), np.pi ** 2)
-    for d in (5, 4):
-        fit_rel_minimax(f"WQ{d}", wq, 0.0, 0.25, d)
+    fit_rel_minimax("WQ5", wq, 0.0, 0.25, 5)
     # sin(pi*a) = a * SP(x), x = a^2, a in [0, 1/2]
     fit("SP", lambda x: (mp.sin(pi * mp.sqrt(x)) / mp.sqrt(x)) if x > 0 else pi, 0.0, 0.25, 5)
     # sin(2*pi*b) = b * SN(x), x = b^2, b in [0, 1/4]

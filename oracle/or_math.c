@@ -158,18 +158,16 @@ static const float P3F[21] = {1.f, 3.f, 9.f, 27.f, 81.f, 243.f, 729.f, 2187.f, 6
 
 /* sin^2(pi t) = x Q_d(x), x = t^2, |t| <= 1/2: Q_d of degree d fitted to
  * sin^2(pi sqrt(x)) / x on [0, 1/4] (least squares reweighted toward the minimax relative
- * error; fit_coeffs.py): the engine's tables (sg-emtf_amd/csrc/emtf_math.h kWQ5, kWQ4). */
+ * error; fit_coeffs.py): the engine's table (sg-emtf_amd/csrc/emtf_math.h kWQ5), d = 5. */
 static const float WQ5[6] = {9.869604110717773f, -32.469661712646484f, 42.726924896240234f, -30.098546981811523f,
                              13.03709602355957f, -3.336198329925537f};
-static const float WQ4[5] = {9.86959457397461f, -32.46799087524414f, 42.67863464355469f, -29.595659255981445f,
-                             10.860730171203613f};
-
 /* Fractional parts of 3^k |z| for 2^-32 <= |z| < 2^23 are carried as 0.64
- * fixed-point words y (frac = y / 2^64), so 3^k |z| mod 1 is y_{k+1} = 3 y_k mod 2^64
- * exactly; each term reads the top 32 bits as a signed word q, t = q 2^-32 in
+ * fixed-point words y (frac = y / 2^64), so 3^(k+2) |z| mod 1 is y_{k+2} = 9 y_k mod 2^64
+ * exactly; an even term k reads the top 32 bits as a signed word q, t = q 2^-32 in
  * [-1/2, 1/2) (q correctly rounded to binary32): t = frac or frac - 1, so
- * sin^2(pi t) = sin^2(pi frac).  Term k is 2^(1-k) x Q_d(x) with the weight folded into
- * the coefficients (exact), d = 5 for k < 2, 4 after. */
+ * s = sin^2(pi t) = sin^2(pi frac) = x Q_5(x), with half the term's weight folded into
+ * the coefficients (exact).  The odd term k + 1 is 2^-k s (3 - 4 s)^2 by the triple-angle
+ * identity sin^2(3a) = sin^2(a) (3 - 4 sin^2(a))^2; terms are added in increasing k. */
 float or_weier_term(float z) {
     if (z != z) return z;
     float a = fabsf(z);
@@ -188,17 +186,20 @@ float or_weier_term(float z) {
     uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
     int s = 23 - ex;                            /* fraction bits, 1..55 */
     uint64_t y = m << (64 - s);                 /* integer part shifted out */
-    for (int k = 0; k <= 20; ++k) {
+    for (int k = 0; k <= 20; k += 2) {
         int32_t q = (int32_t)(uint32_t)(y >> 32);
         float t = (float)q * 2.3283064365386963e-10f;
         float x = t * t;
-        float w = pow2i(1 - k);
-        const float* c = k < 2 ? WQ5 : WQ4;
-        int d = k < 2 ? 5 : 4;
-        float p = c[d] * w;
-        for (int i = d - 1; i >= 0; --i) p = fmaf(p, x, c[i] * w);
-        acc = fmaf(x, p, acc);
-        y = y + (y << 1);
+        float hw = pow2i(-k); /* half the weight of term k */
+        float c = WQ5[5] * hw;
+        for (int i = 4; i >= 0; --i) c = fmaf(c, x, WQ5[i] * hw);
+        float p = x * c;             /* 2^-k s, s = sin^2(pi 3^k z) */
+        acc = fmaf(2.0f, p, acc);    /* term k: 2^(1-k) s */
+        if (k == 20) break;
+        float u = fmaf(p, -pow2i(k + 2), 3.0f); /* 3 - 4 s */
+        float v = u * u;
+        acc = fmaf(p, v, acc);       /* term k + 1: 2^-k s (3 - 4 s)^2 = 2^-k sin^2(pi 3^(k+1) z) */
+        y = y + (y << 3);            /* 9 y mod 2^64 */
     }
     return acc;
 }

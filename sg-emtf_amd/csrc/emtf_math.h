@@ -191,20 +191,56 @@ __device__ __forceinline__ float pow_f(float x, float y) {
 // sin^2(pi t) = x Q_d(x), x = t^2, t in [-1/2, 1/2]: Q_d of degree d fitted to
 // sin^2(pi sqrt(x)) / x on [0, 1/4] by least squares reweighted toward the minimax of the
 // relative error (Lawson's iteration; oracle/fit_coeffs.py); largest relative error of the
-// binary32 Horner form: d = 5: 1.9e-7, d = 4: 1.2e-6 (also at x -> 0, so a sum of terms
-// keeps its relative accuracy near the optimum).
+// binary32 Horner form: d = 5: 1.9e-7 (also at x -> 0, so a sum of terms keeps its
+// relative accuracy near the optimum).
 constexpr float kWQ5[6] = {9.869604110717773f, -32.469661712646484f, 42.726924896240234f, -30.098546981811523f,
                            13.03709602355957f, -3.336198329925537f};
-constexpr float kWQ4[5] = {9.86959457397461f, -32.46799087524414f, 42.67863464355469f, -29.595659255981445f,
-                           10.860730171203613f};
 
 // Weierstrass term sum_{k=0}^{20} 2^(1-k) sin^2(pi 3^k z).  For 2^-32 <= |z| < 2^23
-// the fractional part of 3^k |z| is carried exactly as a 0.64 fixed-point word
-// (3x mod 1 is one 64-bit shift-add); each term reads its top 32 bits as a signed
-// word: t = q 2^-32 in [-1/2, 1/2) differs from the fraction by 0 or 1, so
-// sin^2(pi t) is the term.  Term k is 2^(1-k) x Q_d(x), the weight folded into the
-// coefficients (exact: powers of two), with d = 5 for the two heaviest terms and d = 4
-// for the others: relative error 1.2e-6 at most per term and for the sum.
+// the fractional part of 3^k |z| is carried exactly as a 0.64 fixed-point word (9x mod 1,
+// two terms on, is one 64-bit shift-add).  An even term k reads the word's top 32 bits as
+// a signed word: t = q 2^-32 in [-1/2, 1/2) differs from the fraction by 0 or 1, so
+// s = sin^2(pi t) = x Q_5(x), the weight folded into the coefficients (exact: powers of
+// two).  The odd term k + 1 follows by the triple-angle identity sin^2(3a) = s (3 - 4 s)^2
+// in three operations: its absolute error is at most 9 times that of s and it weighs
+// half as much, so each pair of terms keeps its relative error within about 4 times the
+// polynomial's.  Against 40-digit sums: 6.1e-7 relative at most over 4000 points (1.1e-6
+// with every term from a degree-4 polynomial of its own).
+//
+// The fixed-point path of weier_term for a = |z| in [2^-32, 2^23) (ok: a is), without
+// branches; +0 where ok is false, which is the term for a = 0 and a >= 2^23 (not for NaN
+// or 0 < a < 2^-32, which weier_term sends to its small-|z| branch).
+__device__ __forceinline__ float weier_core(float a, bool ok) {
+    const uint32_t u = __float_as_uint(a);
+    const int ex = (int)((u >> 23) & 0xFFu) - 127;
+    const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
+    uint64_t y = m << ((41 + ex) & 63);  // 64 - (23 - ex) fraction bits: 9 .. 63 where ok
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k <= 20; k += 2) {
+        // the conversion of the high word as one 32-bit instruction (left to itself the
+        // backend widens it to a 64-bit conversion)
+        float qf;
+        asm("v_cvt_f32_i32 %0, %1" : "=v"(qf) : "v"((uint32_t)(y >> 32)));
+        const float t = qf * 2.3283064365386963e-10f;
+        const float x = t * t;
+        const float hw = pow2i(-k);  // half the weight of term k
+        float c = kWQ5[5] * hw;
+#pragma unroll
+        for (int i = 4; i >= 0; --i) c = fmaf(c, x, kWQ5[i] * hw);
+        const float p = x * c;          // 2^-k s, s = sin^2(pi 3^k z)
+        acc = fmaf(2.0f, p, acc);       // term k: 2^(1-k) s
+        if (k == 20) break;
+        const float u = fmaf(p, -pow2i(k + 2), 3.0f);  // 3 - 4 s
+        const float v = u * u;
+        acc = fmaf(p, v, acc);  // term k + 1: 2^-k s (3 - 4 s)^2 = 2^-k sin^2(pi 3^(k+1) z)
+        // y <- 9 y mod 2^64 in one VALU op (the backend otherwise emits 32-bit multiplies
+        // and register moves per step)
+        asm("v_lshl_add_u64 %0, %1, 3, %1" : "=v"(y) : "v"(y));
+    }
+    return ok ? acc : 0.0f;
+}
+
 __device__ __forceinline__ float weier_term(float z) {
     if (z != z) return z;
     const float a = __builtin_fabsf(z);
@@ -223,36 +259,7 @@ __device__ __forceinline__ float weier_term(float z) {
         }
         return acc;
     }
-    const uint32_t u = __float_as_uint(a);
-    const int ex = (int)((u >> 23) & 0xFFu) - 127;
-    const uint64_t m = (uint64_t)((u & 0x007FFFFFu) | 0x00800000u);
-    const int s = 23 - ex;
-    uint64_t y = m << (64 - s);
-#pragma unroll
-    for (int k = 0; k <= 20; ++k) {
-        // the conversion of the high word as one 32-bit instruction (left to itself the
-        // backend widens it to a 64-bit conversion)
-        float qf;
-        asm("v_cvt_f32_i32 %0, %1" : "=v"(qf) : "v"((uint32_t)(y >> 32)));
-        const float t = qf * 2.3283064365386963e-10f;
-        const float x = t * t;
-        const float w = pow2i(1 - k);
-        float q;
-        if (k < 2) {
-            q = kWQ5[5] * w;
-#pragma unroll
-            for (int i = 4; i >= 0; --i) q = fmaf(q, x, kWQ5[i] * w);
-        } else {
-            q = kWQ4[4] * w;
-#pragma unroll
-            for (int i = 3; i >= 0; --i) q = fmaf(q, x, kWQ4[i] * w);
-        }
-        acc = fmaf(x, q, acc);
-        // y <- 3 y mod 2^64 in one VALU op (the backend otherwise emits two 32-bit
-        // multiplies and register moves per step)
-        asm("v_lshl_add_u64 %0, %1, 1, %1" : "=v"(y) : "v"(y));
-    }
-    return acc;
+    return weier_core(a, true);
 }
 
 // ---- the reference's own formulas (evaluation(), SG-EMTF_DE.cu:834-1069) ----------
