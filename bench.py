@@ -69,7 +69,8 @@ def parse():
                     help="convergence workload: unrotated tasks (the CPU configuration: --tasks 8 --pop 64 "
                          "--gens 1000 --interval 10 --elites 2 --no-rotate)")
     ap.add_argument("--no-tune", action="store_true", help="switch the kernel-variant / stream autotuner off")
-    ap.add_argument("--gens", type=int, default=3000, help="polish workload: generations per run")
+    ap.add_argument("--gens", type=int, default=None,
+                    help="polish / convergence workloads: generations per run (default 3000 / 2000)")
     ap.add_argument("--only-func", default=None, help="give every task this objective (per-objective timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters)")
@@ -255,6 +256,7 @@ def bench_polish(a):
     import emtf
     from emtf import tasks as tg
     target, chunk = 1e-6, 50
+    gens = a.gens if a.gens is not None else 3000
     names = ["sphere", "rosenbrock", "ackley", "rastrigin", "griewank", "weierstrass", "schwefel", "elliptic"]
     base = dict(algo=emtf.DE_RAND_1_BIN, F=0.5, CR=0.9, transfer_interval=20, transfer_count=4)
     res = {"plain": [], "polish": []}
@@ -268,7 +270,7 @@ def bench_polish(a):
             p = dict(base, seed=seed)
             if mode == "polish":
                 p.update(polish_interval=50, polish_steps=5, polish_topk=4)
-            t_ms, errs = _converge(tasks, p, a.gens, chunk, target)
+            t_ms, errs = _converge(tasks, p, gens, chunk, target)
             live = funcs != 6
             mean_err = np.nanmean(errs[:, live], axis=1) if live.any() else np.full(errs.shape[0], np.nan)
             res[mode].append(_time_to(t_ms, mean_err, target))
@@ -279,7 +281,7 @@ def bench_polish(a):
                 m = np.mean(errs[:, funcs == f], axis=1)
                 per_func[mode][n].append((_time_to(t_ms, m, target), float(np.mean(errs[-1, funcs == f] <= target)),
                                           float(m[-1])))
-            print(f"seed {seed} {mode}: mean error {mean_err[-1]:.3e} after {a.gens} gens, "
+            print(f"seed {seed} {mode}: mean error {mean_err[-1]:.3e} after {gens} gens, "
                   f"{t_ms[-1]:.0f} ms GPU, time to 1e-6: {res[mode][-1]}", file=sys.stderr, flush=True)
 
     def summary(mode):
@@ -308,11 +310,11 @@ def bench_polish(a):
     ratio = (sq["mean_ms_to_target"] / sp["mean_ms_to_target"]
              if sp["mean_ms_to_target"] and sq["mean_ms_to_target"] else None)
     out = {"metric": "GPU time to mean error 1e-6: adjoint polish / plain DE", "value": ratio, "unit": "ratio",
-           "n_gpus": 1, "steps": a.gens // chunk, "warmup": 0, "higher_is_better": False, "scaling": "weak",
+           "n_gpus": 1, "steps": gens // chunk, "warmup": 0, "higher_is_better": False, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "adjoint polish: 1024 rotated tasks, D=100, pop 128, DE/rand/1/bin F=0.5 CR=0.9, "
                                   "4 elites every 20 gens; polish 5 Armijo steps on the top 4 elites every 50 gens",
-                      "generations": a.gens, "seeds": a.seeds, "target": target,
+                      "generations": gens, "seeds": a.seeds, "target": target,
                       "error": "float64 f(best) - f*, mean over non-Schwefel tasks"},
            "plain": sp, "polish": sq, "per_objective": funcs_out,
            "ratio_on_objectives_reached_by_both": {"objectives": both, "geomean_time_ratio": subset_ratio}}
@@ -331,7 +333,7 @@ def bench_convergence(a):
     import numpy as np
     import emtf
     from emtf import tasks as tg
-    gens = a.gens if a.gens != 3000 else 2000
+    gens = a.gens if a.gens is not None else 2000
     chunk, target = 100, 1e-6
     rows = {n: [] for n in NAMES if n != "schwefel"}
     gpu_ms = []
