@@ -150,6 +150,29 @@ def test_gpu_forward_wide_dims_bit_exact(oracle_mod, D):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("D", [7, 50, 200])
+def test_gpu_weierstrass_term_ranges_bit_exact(oracle_mod, D):
+    """The Weierstrass term at the edges of its branches (z = 0, the 2^-32 bound of the
+    fixed-point path and the small-|z| branch below it, |z| near and past 2^23): zero
+    shift, so z = x exactly; every point mixes the categories over its dimensions."""
+    from emtf.objective import forward_raw
+    lo = np.float32(2.0 ** -32)
+    cats = np.array([0.0, lo, np.nextafter(lo, np.float32(0)), np.nextafter(lo, np.float32(1)), 1e-10, 3e-12,
+                     1e-20, 0.3, 3.7, 0.5, 1.0, 1e6, 8388607.5, 2.0 ** 23, 1e8], np.float32)
+    rng = np.random.default_rng(D)
+    n = 96
+    x = cats[rng.integers(0, len(cats), (n, D))] * rng.choice(np.float32([-1, 1]), (n, D))
+    x[0] = 0.0  # f = 0 exactly
+    x[1] = cats[4]  # every coordinate on the small-|z| branch
+    t = dict(func=5, dim=D, xform=0, shift=np.zeros(D, np.float32))
+    print("case: weierstrass ranges D", D, flush=True)
+    f = forward_raw(5, _cuda(x), None, _cuda(t["shift"])).cpu().numpy()
+    ref = oracle_mod.eval32(t, x)
+    assert ref[0] == 0.0 and ref[1] > 0.0
+    assert np.array_equal(f.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D", [7, 50, 100, 128])
 def test_gpu_adjoint_bit_exact(oracle_mod, D):
     """emtf_objective_adjoint with unit upstream gradient equals the oracle's grad32 bit for bit."""
