@@ -273,6 +273,49 @@ __device__ __forceinline__ float objective(const f16v (&acc)[NT], int D, const f
                 const float v = t1 + b2;
                 if (j + 1 < D) s1 = s1 + v;
             }
+    } else if constexpr (FUNC == F_WEIERSTRASS && SKIP && !ZPAD) {
+        // The wide kernels (SKIP, rows not zero-padded): four rows at a time, their terms
+        // by the branch-free path (independent chains the scheduler interleaves), the rare
+        // arguments redone by the full term, then the row sums in row order as below (s1 is
+        // never -0, so adding +0 for a masked row leaves it).  The LDS epoch kernel keeps
+        // one row at a time: there the interleaved chains take issue slots from the tasks
+        // that share the CU and the mixed epoch is slower.
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                if (32 * jt + 8 * g >= D) continue;
+                float v[4];
+                bool rare = false;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float a = __builtin_fabsf(acc[jt][4 * g + e]);
+                    const bool lo = a >= 2.32830644e-10f, hi = a >= 8388608.0f;
+                    v[e] = weier_core(a, lo && !hi);
+                    rare = rare || (!lo && !hi && a != 0.0f);  // NaN or 0 < |z| < 2^-32
+                }
+                if (rare) {
+                    // one inlined copy of the full term: a rolled loop, the row picked by selects
+#pragma unroll 1
+                    for (int e = 0; e < 4; ++e) {
+                        float z = acc[jt][4 * g];
+                        z = e == 1 ? acc[jt][4 * g + 1] : z;
+                        z = e == 2 ? acc[jt][4 * g + 2] : z;
+                        z = e == 3 ? acc[jt][4 * g + 3] : z;
+                        const float a = __builtin_fabsf(z);
+                        if (z != z || (a > 0.0f && a < 2.32830644e-10f)) {
+                            const float r = weier_term(z);
+#pragma unroll
+                            for (int f = 0; f < 4; ++f) v[f] = e == f ? r : v[f];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int j = 32 * jt + 8 * g + 4 * h + e;
+                    s1 = s1 + (j < Dm ? v[e] : 0.0f);
+                }
+            }
     } else {
         // SKIP: groups of eight rows past the dimension are skipped outright (a
         // uniform branch) for the objectives with transcendental terms -- worth it
