@@ -468,6 +468,48 @@ __device__ __forceinline__ void tile_vals(const f16v& acc, int jt, int D, const 
             v1[r] = rosen_val(acc[r], zn);
             v2[r] = 0.0f;
         }
+    } else if constexpr (FUNC == F_WEIERSTRASS) {
+        // four rows at a time through the branch-free term (as in the wide kernels'
+        // objective), groups of eight rows past D skipped (a uniform branch); v1 = 0 for the
+        // masked rows as row_vals gives
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v1[4 * g + e] = 0.0f;
+                v2[4 * g + e] = 0.0f;
+            }
+            if (32 * jt + 8 * g >= D) continue;
+            float v[4];
+            bool rare = false;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float a = __builtin_fabsf(acc[4 * g + e]);
+                const bool lo = a >= 2.32830644e-10f, hi = a >= 8388608.0f;
+                v[e] = weier_core(a, lo && !hi);
+                rare = rare || (!lo && !hi && a != 0.0f);  // NaN or 0 < |z| < 2^-32
+            }
+            if (rare) {
+#pragma unroll 1
+                for (int e = 0; e < 4; ++e) {
+                    float z = acc[4 * g];
+                    z = e == 1 ? acc[4 * g + 1] : z;
+                    z = e == 2 ? acc[4 * g + 2] : z;
+                    z = e == 3 ? acc[4 * g + 3] : z;
+                    const float a = __builtin_fabsf(z);
+                    if (z != z || (a > 0.0f && a < 2.32830644e-10f)) {
+                        const float r = weier_term(z);
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) v[f] = e == f ? r : v[f];
+                    }
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int j = 32 * jt + 8 * g + 4 * h + e;
+                v1[4 * g + e] = j < D ? v[e] : 0.0f;
+            }
+        }
     } else {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
